@@ -1,0 +1,45 @@
+"""Built-in networks and their table blocks.
+
+``primordial_network`` assembles the 9-species + energy primordial network of
+dengo/primordial_rates.py + dengo/primordial_cooling.py the way
+examples/run_primordial_network.py:16-53 does (every cooling action, every
+reaction), with the temperature grid 1-1e9 K on 1024 log-spaced points.
+"""
+import numpy as np
+
+from . import chemical_network  # noqa: F401  (registers ge, de)
+from .chemical_network import ChemicalNetwork
+from . import primordial_rates, primordial_cooling  # noqa: F401
+
+PRIMORDIAL_COOLING = ("brem", "ceHI", "ceHeI", "ceHeII", "ciHI", "ciHeI",
+                      "ciHeII", "ciHeIS", "compton", "gammah", "gloverabel08",
+                      "h2formation", "reHII", "reHeII1", "reHeII2", "reHeIII")
+PRIMORDIAL_REACTIONS = tuple("k%02d" % i for i in range(1, 24) if i != 20)
+PRIMORDIAL_SPECIES = ("H2I", "H2II", "HI", "HII", "HM", "HeI", "HeII",
+                      "HeIII", "de", "ge")
+
+
+def primordial_network(T_bounds=(1.0, 1e9), n_bins=1024, stop_time=1e12):
+    primordial_rates.setup_primordial()
+    net = ChemicalNetwork(stop_time=stop_time)
+    for name in PRIMORDIAL_COOLING:
+        net.add_cooling(name)
+    for name in PRIMORDIAL_REACTIONS:
+        net.add_reaction(name)
+    net.init_temperature(T_bounds, n_bins)
+    assert tuple(net.species_list()) == PRIMORDIAL_SPECIES
+    return net
+
+
+_TABLE_CACHE = {}
+
+
+def primordial_tables(T_bounds=(1.0, 1e9), n_bins=1024):
+    """(table names, FP64 [45, n_bins] block) for the primordial network."""
+    key = (float(T_bounds[0]), float(T_bounds[1]), int(n_bins))
+    if key not in _TABLE_CACHE:
+        with np.errstate(all="ignore"):
+            net = primordial_network(T_bounds, n_bins)
+            _TABLE_CACHE[key] = (net.table_names(), net.table_block())
+    names, block = _TABLE_CACHE[key]
+    return list(names), block.copy()

@@ -1,0 +1,225 @@
+"""ctypes binding of the HIP library (dengo_amd/lib/libdengo_hip.so).
+
+The GPU path has no CPU fallback: if the library is missing or no GPU is
+visible, every call raises.
+
+``PrimordialSolver`` wraps one ``primordial_data`` struct: it uploads the
+tables produced by the Python front end and runs ``dengo_evolve_primordial``
+semantics on host arrays (``evolve``) or on device-resident torch tensors
+(``evolve_device``).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libdengo_hip.so")
+
+FP64, MIXED = 0, 1
+PER_CELL, PERSISTENT = 0, 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lib = None
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("schedule", ctypes.c_int),
+                ("block", ctypes.c_int), ("grid", ctypes.c_int),
+                ("max_substeps", ctypes.c_int), ("input_is_number", ctypes.c_int),
+                ("reserved", ctypes.c_int * 6)]
+
+
+class CellOutputs(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_void_p), ("nsub", ctypes.c_void_p),
+                ("nrej", ctypes.c_void_p), ("nsweep", ctypes.c_void_p),
+                ("dt", ctypes.c_void_p), ("ttot", ctypes.c_void_p), ("T", ctypes.c_void_p)]
+
+
+# Every symbol include/*.h declares, with its ctypes signature.
+SIGNATURES = {
+    "primordial_setup_data": (ctypes.c_void_p, [ctypes.c_void_p, ctypes.c_void_p]),
+    "primordial_read_rate_tables": (None, [ctypes.c_void_p]),
+    "primordial_read_cooling_tables": (None, [ctypes.c_void_p]),
+    "primordial_free_data": (None, [ctypes.c_void_p]),
+    "dengo_evolve_primordial": (ctypes.c_double, [ctypes.c_double, _dp, ctypes.c_double, _dp,
+                                                  _dp, _dp, ctypes.c_longlong, ctypes.c_void_p]),
+    "calculate_rhs_primordial": (ctypes.c_int, [_dp, _dp, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_void_p]),
+    "calculate_jacobian_primordial": (ctypes.c_int, [_dp, _dp, ctypes.c_int, ctypes.c_int,
+                                                     ctypes.c_void_p]),
+    "primordial_calculate_temperature": (None, [ctypes.c_void_p, _dp, ctypes.c_int,
+                                                ctypes.c_int]),
+    "dengo_default_options": (None, [ctypes.POINTER(Options)]),
+    "dengo_last_error": (ctypes.c_char_p, []),
+    "primordial_alloc_data": (ctypes.c_void_p, []),
+    "primordial_set_tables": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_double, ctypes.c_double]),
+    "primordial_setup_data_from": (ctypes.c_void_p, [ctypes.c_char_p, ctypes.c_void_p,
+                                                     ctypes.c_void_p]),
+    "dengo_primordial_evolve_device": (
+        ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Options), ctypes.c_double,
+                       ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
+                       ctypes.c_void_p, ctypes.c_longlong, ctypes.POINTER(CellOutputs),
+                       _dp, _dp, ctypes.c_void_p]),
+    "dengo_primordial_evolve_host": (
+        ctypes.c_double, [ctypes.c_void_p, ctypes.POINTER(Options), ctypes.c_double, _dp,
+                          ctypes.c_double, _dp, _dp, _dp, ctypes.c_longlong, _ip, _ip, _ip,
+                          _ip, _dp, _dp]),
+    "dengo_primordial_eval": (ctypes.c_longlong, [ctypes.c_void_p, _dp, ctypes.c_longlong,
+                                                  ctypes.c_double, _dp, _dp, _dp, _dp]),
+    "dengo_last_kernel_ms": (ctypes.c_double, []),
+}
+
+
+def load(path=LIB_PATH):
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise RuntimeError("HIP library %s not built: run dengo_amd/build.py" % path)
+        lib = ctypes.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def last_error():
+    return load().dengo_last_error().decode()
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError("%s failed: %s" % (what, last_error()))
+
+
+def _p(a, t=_dp):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def options(precision=FP64, schedule=PERSISTENT, grid=0, max_substeps=50000,
+            input_is_number=False):
+    o = Options()
+    load().dengo_default_options(ctypes.byref(o))
+    o.precision = precision
+    o.schedule = schedule
+    o.grid = grid
+    o.max_substeps = max_substeps
+    o.input_is_number = int(bool(input_is_number))
+    return o
+
+
+def write_table_blob(path, block, T_bounds, z_bounds=(0.0, 0.0)):
+    """Write the DENGOTB1 blob read by <name>_setup_data: 64-byte header
+    (magic, int32 ntables, int32 nT, double T0, T1, z0, z1, padding) then
+    FP64 [ntables][nT]."""
+    block = np.ascontiguousarray(block, dtype="<f8")
+    hdr = bytearray(64)
+    hdr[0:8] = b"DENGOTB1"
+    hdr[8:16] = np.array([block.shape[0], block.shape[1]], dtype="<i4").tobytes()
+    hdr[16:48] = np.array([T_bounds[0], T_bounds[1], z_bounds[0], z_bounds[1]],
+                          dtype="<f8").tobytes()
+    with open(path, "wb") as f:
+        f.write(bytes(hdr))
+        f.write(block.tobytes())
+
+
+class PrimordialSolver(object):
+    """One primordial_data struct with its tables resident on the GPU."""
+
+    N = 10
+
+    def __init__(self, tables=None, T_bounds=(1.0, 1e9)):
+        lib = load()
+        if tables is None:
+            from .networks import primordial_tables
+            _, tables = primordial_tables(T_bounds)
+        tables = np.ascontiguousarray(tables, dtype=np.float64)
+        self.tables = tables
+        self.T_bounds = T_bounds
+        self.data = lib.primordial_alloc_data()
+        if not self.data:
+            raise MemoryError("primordial_alloc_data")
+        _check(lib.primordial_set_tables(self.data, _p(tables), tables.shape[0],
+                                         tables.shape[1], T_bounds[0], T_bounds[1]),
+               "primordial_set_tables")
+
+    def close(self):
+        if self.data:
+            load().primordial_free_data(self.data)
+            self.data = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def evolve(self, inp, rtol, atol, dtf, dt=-1.0, z=-1.0, **opt):
+        """dengo_evolve semantics on host copies; returns a dict like the
+        oracle's (state, atol, ttot, dt, per-cell status/nsub/nrej/nsweep/dt/T)."""
+        lib = load()
+        out = np.array(inp, dtype=np.float64, copy=True, order="C")
+        at = np.array(atol, dtype=np.float64, copy=True, order="C")
+        rt = np.array(rtol, dtype=np.float64, copy=True, order="C")
+        n = out.shape[0]
+        res = dict(status=np.zeros(n, np.int32), nsub=np.zeros(n, np.int32),
+                   nrej=np.zeros(n, np.int32), nsweep=np.zeros(n, np.int32),
+                   dt_cell=np.zeros(n), T=np.zeros(n))
+        o = options(**opt)
+        dtc = ctypes.c_double(dt)
+        ttot = lib.dengo_primordial_evolve_host(
+            self.data, ctypes.byref(o), dtf, ctypes.byref(dtc), z, _p(out), _p(rt), _p(at), n,
+            _p(res["status"], _ip), _p(res["nsub"], _ip), _p(res["nrej"], _ip),
+            _p(res["nsweep"], _ip), _p(res["dt_cell"]), _p(res["T"]))
+        if ttot < 0:
+            raise RuntimeError("evolve failed: %s" % last_error())
+        res.update(state=out, atol=at, ttot=ttot, dt=dtc.value,
+                   kernel_ms=lib.dengo_last_kernel_ms())
+        return res
+
+    def evolve_device(self, inp, rtol, atol, dtf, dt=-1.0, z=-1.0, outputs=None,
+                      stream=None, sync_min=True, **opt):
+        """In-place integration of device-resident float64 torch tensors
+        [ncells, 10].  outputs: optional dict of device tensors (status, nsub,
+        nrej, nsweep: int32; dt, ttot, T: float64).  Returns (ttot_min, dt_min)
+        when sync_min, else None (asynchronous on `stream`)."""
+        lib = load()
+        for t in (inp, rtol, atol):
+            if not (t.is_cuda and t.dtype.is_floating_point and t.element_size() == 8
+                    and t.is_contiguous() and t.dim() == 2 and t.shape[1] == self.N):
+                raise ValueError("expected contiguous float64 cuda tensors [n, %d]" % self.N)
+        n = inp.shape[0]
+        co = CellOutputs()
+        for k, v in (outputs or {}).items():
+            setattr(co, k, v.data_ptr())
+        o = options(**opt)
+        tmin = ctypes.c_double(0.0)
+        dmin = ctypes.c_double(0.0)
+        s = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+        rc = lib.dengo_primordial_evolve_device(
+            self.data, ctypes.byref(o), dtf, dt, z, ctypes.c_void_p(inp.data_ptr()),
+            ctypes.c_void_p(rtol.data_ptr()), ctypes.c_void_p(atol.data_ptr()), n,
+            ctypes.byref(co), ctypes.byref(tmin) if sync_min else None,
+            ctypes.byref(dmin) if sync_min else None, s)
+        _check(rc, "dengo_primordial_evolve_device")
+        return (tmin.value, dmin.value) if sync_min else None
+
+    def eval(self, u, z=-1.0, jacobian=True):
+        """RHS f[n,10], Jacobian J[n,10,10] (J[c, col, row]), T[n], rates[n,45]
+        and the count of cells whose RHS saw a negative value."""
+        lib = load()
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        n = u.shape[0]
+        f = np.zeros((n, 10))
+        J = np.zeros((n, 10, 10)) if jacobian else None
+        T = np.zeros(n)
+        rates = np.zeros((n, self.tables.shape[0]))
+        r = lib.dengo_primordial_eval(self.data, _p(u), n, z, _p(f), _p(J), _p(T), _p(rates))
+        if r < 0:
+            raise RuntimeError("eval failed: %s" % last_error())
+        return f, J, T, rates, int(r)

@@ -1,0 +1,715 @@
+/*
+ * CPU FP64 oracle for the Dengo backward-Euler chemistry solver.
+ *
+ * TEST INFRASTRUCTURE ONLY: a plain-C restatement of the reference algorithm,
+ * used by tests/, the smoke check and bench.py's CPU-baseline leg as the
+ * checker.  The product (dengo_amd/) never links, loads or calls it.
+ *
+ * Every function cites the reference lines it restates.  Compile with
+ * -ffp-contract=off so that each expression rounds exactly as written.
+ */
+#include "dengo_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Network layout                                                            */
+/* ------------------------------------------------------------------------ */
+/* Unknowns sorted by name (rates_and_rate_tables.C.template uses
+ * `required_species | sort`, ordered by repr "Species: <name>"). */
+enum { H2I, H2II, HI, HII, HM, HeI, HeII, HeIII, DE, GE };
+
+/* Tables: reactions k01..k19,k21..k23, then cooling tables, both in the
+ * emitted struct's dictsort order (rates_and_rate_tables.h.template:57-80). */
+enum {
+    K01, K02, K03, K04, K05, K06, K07, K08, K09, K10, K11, K12, K13, K14, K15,
+    K16, K17, K18, K19, K21, K22, K23,
+    T_BREM, T_CEHEI, T_CEHEII, T_CEHI, T_CIHEI, T_CIHEII, T_CIHEIS, T_CIHI,
+    T_COMP, T_GAMMAH, T_GAEL, T_GAH2, T_GAHE, T_GAHI, T_GAHP, T_GPHDL, T_GPLDL,
+    T_H2MCOOL, T_H2MHEAT, T_REHEII1, T_REHEII2, T_REHEIII, T_REHII
+};
+
+/* Species weights (sorted order) and free electrons
+ * (dengo/periodictabledump.csv:1-2, dengo/primordial_rates.py:36-43). */
+static const double W[ORC_PRIM_N] = {2.0, 2.0, 1.00794, 1.00794, 1.00794,
+                                     4.002602, 4.002602, 4.002602, 1.0, 1.0};
+static const double FREE_E[ORC_PRIM_N] = {0.0, 1.0, 0.0, 1.0, -1.0,
+                                          0.0, 1.0, 2.0, 0.0, 0.0};
+
+/* Reactions as reactant slots and product (count, species) pairs
+ * (dengo/primordial_rates.py:47-293). */
+typedef struct { int table, nslot, slot[3], nprod, pc[2], ps[2]; } rxn_t;
+static const rxn_t RXN[22] = {
+    {K01, 2, {HI, DE, 0}, 2, {1, 2}, {HII, DE}},
+    {K02, 2, {HII, DE, 0}, 1, {1, 0}, {HI, 0}},
+    {K03, 2, {HeI, DE, 0}, 2, {1, 2}, {HeII, DE}},
+    {K04, 2, {HeII, DE, 0}, 1, {1, 0}, {HeI, 0}},
+    {K05, 2, {HeII, DE, 0}, 2, {1, 2}, {HeIII, DE}},
+    {K06, 2, {HeIII, DE, 0}, 1, {1, 0}, {HeII, 0}},
+    {K07, 2, {HI, DE, 0}, 1, {1, 0}, {HM, 0}},
+    {K08, 2, {HM, HI, 0}, 2, {1, 1}, {H2I, DE}},
+    {K09, 2, {HI, HII, 0}, 1, {1, 0}, {H2II, 0}},
+    {K10, 2, {H2II, HI, 0}, 2, {1, 1}, {H2I, HII}},
+    {K11, 2, {H2I, HII, 0}, 2, {1, 1}, {H2II, HI}},
+    {K12, 2, {H2I, DE, 0}, 2, {2, 1}, {HII, DE}},
+    {K13, 2, {H2I, HI, 0}, 1, {3, 0}, {HI, 0}},
+    {K14, 2, {HM, DE, 0}, 2, {1, 2}, {HI, DE}},
+    {K15, 2, {HM, HI, 0}, 2, {2, 1}, {HI, DE}},
+    {K16, 2, {HM, HII, 0}, 1, {2, 0}, {HI, 0}},
+    {K17, 2, {HM, HII, 0}, 2, {1, 1}, {H2II, DE}},
+    {K18, 2, {H2II, DE, 0}, 1, {2, 0}, {HI, 0}},
+    {K19, 2, {H2II, HM, 0}, 2, {1, 1}, {HI, H2I}},
+    {K21, 3, {HI, HI, H2I}, 1, {2, 0}, {H2I, 0}},
+    {K22, 3, {HI, HI, HI}, 2, {1, 1}, {H2I, HI}},
+    {K23, 2, {H2I, H2I, 0}, 2, {2, 1}, {HI, H2I}},
+};
+
+/* net change of species s in reaction r: products minus reactants
+ * (dengo/reaction_classes.py:113-116) */
+static double net_change(const rxn_t *r, int s)
+{
+    int up = 0, down = 0;
+    for (int k = 0; k < r->nprod; ++k) if (r->ps[k] == s) up += r->pc[k];
+    for (int k = 0; k < r->nslot; ++k) if (r->slot[k] == s) down += 1;
+    return (double)(up - down);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Tables (dengo/primordial_rates.py, dengo/primordial_cooling.py)           */
+/* ------------------------------------------------------------------------ */
+static double poly_exp(double x, const double *c, int n)
+{
+    double acc = c[0];
+    for (int k = 1; k < n; ++k) acc = acc + c[k] * pow(x, (double)k);
+    return exp(acc);
+}
+
+static double poly_pow10(double x, const double *c, int n)
+{
+    double acc = c[0];
+    for (int k = 1; k < n; ++k) acc = acc + c[k] * pow(x, (double)k);
+    return pow(10.0, acc);
+}
+
+static double capped_boltz(double T, double e)
+{
+    double a = e / T, cap = log(1.0e30);
+    return exp(-(a < cap ? a : cap));
+}
+
+void orc_primordial_tables(const double *Tv, int n, double *out)
+{
+    static const double c01[9] = {-32.71396786375, 13.53655609057, -5.739328757388,
+        1.563154982022, -0.2877056004391, 0.03482559773736999, -0.00263197617559,
+        0.0001119543953861, -2.039149852002e-6};
+    static const double c02[10] = {-28.61303380689232, -0.7241125657826851,
+        -0.02026044731984691, -0.002380861877349834, -0.0003212605213188796,
+        -0.00001421502914054107, 4.989108920299513e-6, 5.755614137575758e-7,
+        -1.856767039775261e-8, -3.071135243196595e-9};
+    static const double c03[9] = {-44.09864886561001, 23.91596563469, -10.75323019821,
+        3.058038757198, -0.5685118909884001, 0.06795391233790001, -0.005009056101857001,
+        0.0002067236157507, -3.649161410833e-6};
+    static const double c05[9] = {-68.71040990212001, 43.93347632635, -18.48066993568,
+        4.701626486759002, -0.7692466334492, 0.08113042097303, -0.005324020628287001,
+        0.0001975705312221, -3.165581065665e-6};
+    static const double c08[8] = {-20.06913897587003, 0.2289800603272916,
+        0.03599837721023835, -0.004555120027032095, -0.0003105115447124016,
+        0.0001073294010367247, -8.36671960467864e-6, 2.238306228891639e-7};
+    static const double c11[9] = {-24.24914687731536, 3.400824447095291,
+        -3.898003964650152, 2.045587822403071, -0.5416182856220388, 0.0841077503763412,
+        -0.007879026154483455, 0.0004138398421504563, -9.36345888928611e-6};
+    static const double c14[9] = {-18.01849334273, 2.360852208681, -0.2827443061704,
+        0.01623316639567, -0.03365012031362999, 0.01178329782711, -0.001656194699504,
+        0.0001068275202678, -2.631285809207e-6};
+    static const double c15[10] = {-20.37260896533324, 1.139449335841631,
+        -0.1421013521554148, 0.00846445538663, -0.0014327641212992, 0.0002012250284791,
+        0.0000866396324309, -0.00002585009680264, 2.4555011970392e-6, -8.06838246118e-8};
+    static const double gaHI_hi[6] = {-24.311209, 4.6450521, -3.7209846, 5.9369081,
+                                      -5.5108047, 1.5538288};
+    static const double gaHI_mid[6] = {-24.311209, 3.5692468, -11.332860, -27.850082,
+                                       -21.328264, -4.2519023};
+    static const double gaHI_lo[6] = {-16.818342, 37.383713, 58.145166, 48.656103,
+                                      20.159831, 3.8479610};
+    static const double gaH2c[6] = {-23.962112, 2.09433740, -0.77151436, 0.43693353,
+                                    -0.14913216, -0.033638326};
+    static const double gaHec[6] = {-23.689237, 2.1892372, -0.81520438, 0.29036281,
+                                    -0.16596184, 0.19191375};
+    static const double gaHpc[6] = {-21.716699, 1.3865783, -0.37915285, 0.11453688,
+                                    -0.23214154, 0.058538864};
+    static const double gael_hi[6] = {-22.190316, 1.5728955, -0.21335100, 0.96149759,
+                                      -0.91023195, 0.13749749};
+    static const double gael_lo[6] = {-34.286155, -48.537163, -77.121176, -51.352459,
+                                      -15.169160, -0.98120322};
+    const double tevk = 1.1605e4, tiny = 1e-30;
+
+    for (int i = 0; i < n; ++i) {
+        const double T = Tv[i], tev = T / tevk, L = log(tev);
+        double *o = out + i;
+#define SET(t, v) o[(size_t)(t) * n] = (v)
+        double k01 = poly_exp(L, c01, 9);
+        SET(K01, k01);
+        SET(K02, tev < 0.8 ? 3.92e-13 / pow(tev, 0.6353) : poly_exp(L, c02, 10));
+        double k03 = tev > 0.8 ? poly_exp(L, c03, 9) : tiny;
+        SET(K03, k03);
+        SET(K04, tev > 0.8 ? 1.54e-9 * (1. + 0.3 / exp(8.099328789667 / tev))
+                               / (exp(40.49664394833662 / tev) * pow(tev, 1.5))
+                               + 3.92e-13 / pow(tev, 0.6353)
+                           : tiny);
+        double k05 = tev > 0.8 ? poly_exp(L, c05, 9) : tiny;
+        SET(K05, k05);
+        SET(K06, 3.36e-10 / sqrt(T) / pow(T / 1.e3, 0.2) / (1 + pow(T / 1.e6, 0.7)));
+        SET(K07, 6.77e-15 * pow(tev, 0.8779));
+        SET(K08, tev > 0.1 ? poly_exp(L, c08, 8) : 1.43e-9);
+        SET(K09, T > 6.7e3 ? 5.81e-16 * pow(T / 56200, -0.6657 * log10(T / 56200))
+                           : 1.85e-23 * pow(T, 1.8));
+        SET(K10, 6.0e-10);
+        SET(K11, tev > 0.3 ? poly_exp(L, c11, 9) : tiny);
+        SET(K12, 5.6e-11 * exp(-102124 / T) * pow(T, 0.5));
+        double lt = log10(T);
+        double k13 = pow(10.0, -178.4239 - 68.42243 * lt + 43.20243 * lt * lt
+                                - 4.633167 * lt * lt * lt
+                                + 69.70086 * log10(1 + 40870.38 / T) - 23705.7 / T);
+        SET(K13, k13);
+        SET(K14, tev > 0.04 ? poly_exp(L, c14, 9) : tiny);
+        SET(K15, tev > 0.1 ? poly_exp(L, c15, 10) : 2.56e-9 * pow(tev, 1.78186));
+        SET(K16, 6.5e-9 / sqrt(tev));
+        SET(K17, T < 1e4 ? 1.0e-8 * pow(T, -0.4) : 4.0e-4 * pow(T, -1.4) * exp(-15100.0 / T));
+        SET(K18, T > 617 ? 1.32e-6 * pow(T, -0.76) : 1.e-8);
+        SET(K19, 5.e-7 * sqrt(100. / T));
+        SET(K21, 2.8e-31 * pow(T, -0.6));
+        double k22 = 7.7e-31 / pow(T, 0.464);
+        SET(K22, k22);
+        SET(K23, 8.125e-8 / sqrt(T) * exp(-52000 / T) * (1.0 - exp(-6000 / T)));
+
+        const double sq5 = 1.0 + sqrt(T / 1.0e5);
+        SET(T_BREM, 1.43e-27 * sqrt(T) * (1.1 + 0.34 * exp(-pow(5.5 - log10(T), 2) / 3.0)));
+        SET(T_CEHEI, 9.1e-27 * capped_boltz(T, 13179.0) * pow(T, -0.1687) / sq5);
+        SET(T_CEHEII, 5.54e-17 * capped_boltz(T, 473638.0) * pow(T, -0.397) / sq5);
+        SET(T_CEHI, 7.5e-19 * capped_boltz(T, 118348.0) / sq5);
+        SET(T_CIHEI, 3.94e-11 * k03);
+        SET(T_CIHEII, 8.72e-11 * k05);
+        SET(T_CIHEIS, 5.01e-27 * pow(T, -0.1687) / sq5 * capped_boltz(T, 55338.0));
+        SET(T_CIHI, 2.18e-11 * k01);
+        SET(T_COMP, 5.65e-36);
+        SET(T_GAMMAH, 8.5e-26 + T);
+
+        double tm = T < 10.0 ? 10.0 : (T > 1.e4 ? 1.e4 : T);
+        double lt3 = log10(tm / 1.e3);
+        double gaHI = T < 100.0 ? poly_pow10(lt3, gaHI_lo, 6)
+                    : (T < 1000.0 ? poly_pow10(lt3, gaHI_mid, 6) : poly_pow10(lt3, gaHI_hi, 6));
+        SET(T_GAEL, T < 200 ? poly_pow10(lt3, gael_lo, 6) : poly_pow10(lt3, gael_hi, 6));
+        SET(T_GAH2, poly_pow10(lt3, gaH2c, 6));
+        SET(T_GAHE, poly_pow10(lt3, gaHec, 6));
+        SET(T_GAHI, gaHI);
+        SET(T_GAHP, poly_pow10(lt3, gaHpc, 6));
+        double t3 = tm / 1000.;
+        double hdlr = (9.5e-22 * pow(t3, 3.76)) / (1. + 0.12 * pow(t3, 2.1))
+                      * exp(-pow(0.13 / t3, 3)) + 3.e-24 * exp(-0.51 / t3);
+        double hdlv = 6.7e-19 * exp(-5.86 / t3) + 1.6e-18 * exp(-11.7 / t3);
+        SET(T_GPHDL, hdlr + hdlv);
+        double tg = T < 13.0 ? 13.0 : (T > 1e5 ? 1e5 : T);
+        double lg = log10(tg);
+        SET(T_GPLDL, pow(10., -103.0 + 97.59 * lg - 48.05 * lg * lg + 10.80 * lg * lg * lg
+                              - 0.9032 * lg * lg * lg * lg));
+        SET(T_H2MCOOL, 7.177e-12 * k13);
+        SET(T_H2MHEAT, 7.177e-12 * k22);
+        SET(T_REHEII1, 1.55e-26 * pow(T, 0.3647));
+        SET(T_REHEII2, 1.24e-13 * pow(T, -1.5) * capped_boltz(T, 470000.0)
+                           * (1. + 0.3 * capped_boltz(T, 94000.0)));
+        SET(T_REHEIII, 3.48e-26 * sqrt(T) * pow(T / 1000.0, -0.2) / (1.0 + pow(T / 1.0e6, 0.7)));
+        SET(T_REHII, 8.70e-27 * sqrt(T) * pow(T / 1000.0, -0.2) / (1.0 + pow(T / 1.0e6, 0.7)));
+#undef SET
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Portable natural log (ORC_FLAG_PORTABLE_LOG).                             */
+/* The product's device code takes ln T with a log built from IEEE + - * /   */
+/* and an exact frexp (|error| < 0.71 ulp): x = 2^e m, m in [sqrt(1/2),      */
+/* sqrt(2)), f = m - 1, s = f/(2+f), ln(1+f) = f - s (f - P(s^2)) with the   */
+/* atanh series P(z) = sum_{k=1..9} 2 z^k/(2k+1), e ln2 split hi/lo.  The    */
+/* same evaluation here lets a test compare GPU and CPU bit for bit; the     */
+/* default remains libm's log as the reference uses.                        */
+/* ------------------------------------------------------------------------ */
+double orc_plog(double x)
+{
+    if (x != x) return x;
+    if (x <= 0.0) return x == 0.0 ? -HUGE_VAL : NAN;
+    if (x > 1.7976931348623157e308) return x;
+    int e;
+    double m = frexp(x, &e);
+    if (m < 0.7071067811865476) { m = m * 2.0; e = e - 1; }
+    double f = m - 1.0, s = f / (2.0 + f), z = s * s;
+    static const double c[9] = {0.6666666666666666, 0.4, 0.2857142857142857,
+        0.2222222222222222, 0.18181818181818182, 0.15384615384615385,
+        0.13333333333333333, 0.11764705882352941, 0.10526315789473684};
+    double P = c[8];
+    for (int k = 7; k >= 0; --k) P = c[k] + z * P;
+    P = z * P;
+    double de = (double)e;
+    double hi = de * 0.6931471803691238, lo = de * 1.9082149292705877e-10;
+    double r = f - (s * (f - P) - lo);
+    return hi + r;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Temperature, interpolation, RHS, Jacobian                                 */
+/* ------------------------------------------------------------------------ */
+void orc_prim_setup(orc_prim_data *d, const double *tables, int nT, double T0, double T1)
+{
+    memset(d, 0, sizeof(*d));
+    /* rates_and_rate_tables.C.template:25-29 */
+    d->bounds[0] = T0;
+    d->bounds[1] = T1;
+    d->nbins = nT - 1;
+    d->dbin = (log(d->bounds[1]) - log(d->bounds[0])) / d->nbins;
+    d->idbin = (double)(1.0L / d->dbin);
+    d->lb = log(d->bounds[0]);
+    d->tables = tables;
+    d->nT = nT;
+}
+
+/* rates_and_rate_tables.C.template:309-350 with chemical_network.py:170-207:
+ * density = sum of A_s n_s over species other than ge, de;
+ * T = density*ge*mh / (kb * sum_{s != ge} n_s/(gamma_s - 1)), clamped. */
+void orc_prim_temperature(orc_prim_data *d, const double *u)
+{
+    const double kb = 1.3806504e-16, mh = 1.67e-24;
+    const double gamma = 5.e0 / 3.e0, gammaH2 = 7.e0 / 5.e0;
+    double density = 2.0 * u[H2I] + 2.0 * u[H2II] + 1.00794 * u[HI] + 1.00794 * u[HII]
+                   + 1.00794 * u[HM] + 4.002602 * u[HeI] + 4.002602 * u[HeII]
+                   + 4.002602 * u[HeIII];
+    double gsum = u[H2I] / (gammaH2 - 1.0) + u[H2II] / (gammaH2 - 1.0)
+                + u[HI] / (gamma - 1.0) + u[HII] / (gamma - 1.0) + u[HM] / (gamma - 1.0)
+                + u[HeI] / (gamma - 1.0) + u[HeII] / (gamma - 1.0)
+                + u[HeIII] / (gamma - 1.0) + u[DE] / (gamma - 1.0);
+    double T = density * u[GE] * mh / (kb * gsum);
+    if (T < d->bounds[0]) T = d->bounds[0];
+    else if (T > d->bounds[1]) T = d->bounds[1];
+    d->T = T;
+    d->logT = (d->flags & ORC_FLAG_PORTABLE_LOG) ? orc_plog(T) : log(T);
+    d->invT = 1.0 / T;
+    d->dTs_ge = density * mh / (kb * gsum);
+}
+
+/* rates_and_rate_tables.C.template:362-411,428-438.  The bin is clamped for
+ * the table lookup but t1/t2 use the unclamped bin (:373-382).  The dk/dT
+ * values (drs_, dcs_) are not formed: the emitted Jacobian never reads them
+ * (its energy column is identically zero). */
+void orc_prim_interpolate(orc_prim_data *d)
+{
+    int bin_id = (int)(d->idbin * (d->logT - d->lb));
+    int b = bin_id;
+    if (b <= 0) b = 0;
+    else if (b >= d->nbins) b = d->nbins - 1;
+    double t1 = (d->lb + (bin_id) * d->dbin);
+    double t2 = (d->lb + (bin_id + 1) * d->dbin);
+    d->bin_id = b;
+    d->Tdef = (d->logT - t1) / (t2 - t1);
+    d->dT = (t2 - t1);
+    for (int t = 0; t < ORC_PRIM_NTAB; ++t) {
+        const double *r = d->tables + (size_t)t * d->nT;
+        d->rs[t] = r[b] + d->Tdef * (r[b + 1] - r[b]);
+    }
+}
+
+/* Cooling sum in action order brem, ceHeI, ceHeII, ceHI, ciHeI, ciHeII,
+ * ciHeIS, ciHI, compton, gammah (identically 0), gloverabel08, h2formation,
+ * reHeII1, reHeII2, reHeIII, reHII (dengo/primordial_cooling.py:34-316). */
+static double cooling(const orc_prim_data *d, const double *u)
+{
+    const double *c = d->rs;
+    const double de = u[DE], zp1 = d->current_z + 1.0;
+    double galdl = c[T_GAHI] * u[HI] + c[T_GAH2] * u[H2I] + c[T_GAHE] * u[HeI]
+                 + c[T_GAHP] * u[HII] + c[T_GAEL] * de;
+    double sum = 0.0;
+    sum += -c[T_BREM] * (u[HII] + u[HeII] + 4.0 * u[HeIII]) * de;
+    sum += -c[T_CEHEI] * u[HeII] * de * de;
+    sum += -c[T_CEHEII] * u[HeII] * de;
+    sum += -c[T_CEHI] * u[HI] * de;
+    sum += -c[T_CIHEI] * u[HeI] * de;
+    sum += -c[T_CIHEII] * u[HeII] * de;
+    sum += -c[T_CIHEIS] * u[HeII] * de * de;
+    sum += -c[T_CIHI] * u[HI] * de;
+    sum += -(c[T_COMP] * pow(zp1, 4)) * (d->T - 2.73 * zp1) * de;
+    sum += -u[H2I] * c[T_GPHDL] / (1.0 + c[T_GPHDL] / galdl);
+    sum += c[T_H2MHEAT] * u[HI] * u[HI] * u[HI] - c[T_H2MCOOL] * u[H2I] * u[HI];
+    sum += -c[T_REHEII1] * u[HeII] * de;
+    sum += -c[T_REHEII2] * u[HeII] * de;
+    sum += -c[T_REHEIII] * u[HeIII] * de;
+    sum += -c[T_REHII] * u[HII] * de;
+    return sum;
+}
+
+/* d(cooling)/d(n_col) for each species column (energy column is 0). */
+static void cooling_grad(const orc_prim_data *d, const double *u, double *g)
+{
+    const double *c = d->rs;
+    const double de = u[DE], zp1 = d->current_z + 1.0;
+    for (int k = 0; k < ORC_PRIM_N; ++k) g[k] = 0.0;
+    /* brem */
+    g[HII] += -c[T_BREM] * de;
+    g[HeII] += -c[T_BREM] * de;
+    g[HeIII] += -4.0 * c[T_BREM] * de;
+    g[DE] += -c[T_BREM] * (u[HII] + u[HeII] + 4.0 * u[HeIII]);
+    /* ceHeI, ciHeIS: -c HeII de^2 */
+    g[HeII] += -c[T_CEHEI] * de * de;
+    g[DE] += -2.0 * c[T_CEHEI] * u[HeII] * de;
+    /* ceHeII */
+    g[HeII] += -c[T_CEHEII] * de;
+    g[DE] += -c[T_CEHEII] * u[HeII];
+    /* ceHI */
+    g[HI] += -c[T_CEHI] * de;
+    g[DE] += -c[T_CEHI] * u[HI];
+    /* ciHeI */
+    g[HeI] += -c[T_CIHEI] * de;
+    g[DE] += -c[T_CIHEI] * u[HeI];
+    /* ciHeII */
+    g[HeII] += -c[T_CIHEII] * de;
+    g[DE] += -c[T_CIHEII] * u[HeII];
+    /* ciHeIS */
+    g[HeII] += -c[T_CIHEIS] * de * de;
+    g[DE] += -2.0 * c[T_CIHEIS] * u[HeII] * de;
+    /* ciHI */
+    g[HI] += -c[T_CIHI] * de;
+    g[DE] += -c[T_CIHI] * u[HI];
+    /* compton */
+    g[DE] += -(c[T_COMP] * pow(zp1, 4)) * (d->T - 2.73 * zp1);
+    /* gloverabel08: C = -H2I g / D, D = 1 + g/L, L = sum ga_x x */
+    {
+        double gp = c[T_GPHDL];
+        double L = c[T_GAHI] * u[HI] + c[T_GAH2] * u[H2I] + c[T_GAHE] * u[HeI]
+                 + c[T_GAHP] * u[HII] + c[T_GAEL] * de;
+        double D = 1.0 + gp / L;
+        double common = -u[H2I] * gp * gp / (L * L * D * D);
+        g[H2I] += -gp / D + common * c[T_GAH2];
+        g[HI] += common * c[T_GAHI];
+        g[HeI] += common * c[T_GAHE];
+        g[HII] += common * c[T_GAHP];
+        g[DE] += common * c[T_GAEL];
+    }
+    /* h2formation */
+    g[HI] += 3.0 * c[T_H2MHEAT] * u[HI] * u[HI] - c[T_H2MCOOL] * u[H2I];
+    g[H2I] += -c[T_H2MCOOL] * u[HI];
+    /* recombination cooling */
+    g[HeII] += -c[T_REHEII1] * de;
+    g[DE] += -c[T_REHEII1] * u[HeII];
+    g[HeII] += -c[T_REHEII2] * de;
+    g[DE] += -c[T_REHEII2] * u[HeII];
+    g[HeIII] += -c[T_REHEIII] * de;
+    g[DE] += -c[T_REHEIII] * u[HeIII];
+    g[HII] += -c[T_REHII] * de;
+    g[DE] += -c[T_REHII] * u[HII];
+}
+
+static double mdensity_of(const double *u)
+{
+    /* unweighted sum over species other than ge/de, times mh
+     * (rates_and_rate_tables.C.template:491-508) */
+    double m = 0.0;
+    for (int s = 0; s < ORC_PRIM_N; ++s)
+        if (s != GE && s != DE) m += u[s];
+    return m * 1.67e-24;
+}
+
+/* calculate_rhs (rates_and_rate_tables.C.template:460-522), nstrip = 1. */
+int orc_prim_rhs(double *u, double *f, int nstrip, int nchem, void *sdata)
+{
+    orc_prim_data *d = (orc_prim_data *)sdata;
+    (void)nstrip; (void)nchem;
+    d->rhs_calls++;
+    orc_prim_temperature(d, u);
+    orc_prim_interpolate(d);
+    for (int s = 0; s < ORC_PRIM_N; ++s)
+        if (u[s] < 0.0) return 1;
+    double md = mdensity_of(u);
+    for (int s = 0; s < ORC_PRIM_N; ++s) f[s] = 0.0;
+    for (int r = 0; r < 22; ++r) {
+        const rxn_t *x = &RXN[r];
+        double flux = d->rs[x->table];
+        for (int k = 0; k < x->nslot; ++k) flux = flux * u[x->slot[k]];
+        for (int s = 0; s < GE; ++s) {
+            double nu = net_change(x, s);
+            if (nu != 0.0) f[s] += nu * flux;
+        }
+    }
+    f[GE] = cooling(d, u) / md;
+    return 0;
+}
+
+/* calculate_jacobian (rates_and_rate_tables.C.template:527-599), nstrip = 1.
+ * J[col*10 + row] = d f_row / d n_col with the rates of the last RHS call
+ * held fixed; the ge row is divided by mdensity; the ge column is zero. */
+int orc_prim_jac(double *u, double *J, int nstrip, int nchem, void *sdata)
+{
+    orc_prim_data *d = (orc_prim_data *)sdata;
+    (void)nstrip; (void)nchem;
+    for (int s = 0; s < ORC_PRIM_N; ++s)
+        if (u[s] < 0.0) return 1;
+    double md = mdensity_of(u);
+    for (int k = 0; k < ORC_PRIM_N * ORC_PRIM_N; ++k) J[k] = 0.0;
+    for (int r = 0; r < 22; ++r) {
+        const rxn_t *x = &RXN[r];
+        for (int j = 0; j < x->nslot; ++j) {
+            /* derivative of the rate law through reactant slot j */
+            double part = d->rs[x->table];
+            for (int k = 0; k < x->nslot; ++k)
+                if (k != j) part = part * u[x->slot[k]];
+            int col = x->slot[j];
+            for (int s = 0; s < GE; ++s) {
+                double nu = net_change(x, s);
+                if (nu != 0.0) J[col * ORC_PRIM_N + s] += nu * part;
+            }
+        }
+    }
+    double g[ORC_PRIM_N];
+    cooling_grad(d, u, g);
+    for (int col = 0; col < ORC_PRIM_N; ++col)
+        J[col * ORC_PRIM_N + GE] = g[col] / md;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Newton / Gauss elimination (dengo/solvers/BE_chem_solve.C)                */
+/* ------------------------------------------------------------------------ */
+#define IDX(i, j, n) ((j) * (n) + (i))
+/* BE_chem_solve.C:329-382: partial pivoting (first strictly larger |a|),
+ * swap of columns k..n-1 only, multipliers stored in place, no singularity
+ * check. */
+int orc_gauss_elim(double *A, double *x, double *b, int n)
+{
+    for (int i = 0; i < n; ++i) x[i] = b[i];
+    for (int k = 0; k < n - 1; ++k) {
+        int p = k;
+        for (int i = k + 1; i < n; ++i)
+            if (fabs(A[IDX(i, k, n)]) > fabs(A[IDX(p, k, n)])) p = i;
+        for (int j = k; j < n; ++j) {
+            double t = A[IDX(k, j, n)];
+            A[IDX(k, j, n)] = A[IDX(p, j, n)];
+            A[IDX(p, j, n)] = t;
+        }
+        double t = x[k];
+        x[k] = x[p];
+        x[p] = t;
+        for (int i = k + 1; i < n; ++i) A[IDX(i, k, n)] /= A[IDX(k, k, n)];
+        for (int j = k + 1; j < n; ++j)
+            for (int i = k + 1; i < n; ++i)
+                A[IDX(i, j, n)] -= A[IDX(i, k, n)] * A[IDX(k, j, n)];
+        for (int i = k + 1; i < n; ++i) x[i] -= A[IDX(i, k, n)] * x[k];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        for (int j = i + 1; j < n; ++j) x[i] -= A[IDX(i, j, n)] * x[j];
+        x[i] /= A[IDX(i, i, n)];
+    }
+    return 0;
+}
+
+/* BE_chem_solve.C:246-275: g(u) = u - u0 - dt f(u) in scaled variables. */
+static int be_resid(orc_rhs_f f, double *u, double *u0, double *gu, double dt, int n,
+                    double *sc, double *inv, void *sdata)
+{
+    for (int i = 0; i < n; ++i) u[i] *= sc[i];
+    if (f(u, gu, 1, n, sdata) != 0) return 1;
+    for (int i = 0; i < n; ++i) u[i] *= inv[i];
+    for (int i = 0; i < n; ++i) gu[i] *= inv[i];
+    for (int i = 0; i < n; ++i) gu[i] = u[i] - u0[i] - dt * gu[i];
+    return 0;
+}
+
+/* BE_chem_solve.C:280-321: J = I - dt * diag(inv) Jf diag(sc). */
+static int be_jac(orc_jac_f J, double *u, double *Ju, double dt, int n,
+                  double *sc, double *inv, void *sdata)
+{
+    for (int i = 0; i < n; ++i) u[i] *= sc[i];
+    if (J(u, Ju, 1, n, sdata) != 0) return 1;
+    for (int i = 0; i < n; ++i) u[i] *= inv[i];
+    for (int c = 0; c < n; ++c)
+        for (int r = 0; r < n; ++r) Ju[c * n + r] *= inv[r];
+    for (int r = 0; r < n; ++r)
+        for (int c = 0; c < n; ++c) Ju[c * n + r] *= sc[c];
+    for (int k = 0; k < n * n; ++k) Ju[k] = -dt * Ju[k];
+    for (int r = 0; r < n; ++r) Ju[r * n + r] += 1.0;
+    return 0;
+}
+
+/* BE_chem_solve.C:79-241 for one cell: <= 10 sweeps, lambda = 1, max-norm
+ * test in scaled units that stops at the first unconverged entry, NaN exit. */
+int orc_BE_chem_solve(orc_rhs_f f, orc_jac_f J, double *u, double dt, double *rtol,
+                      double *atol, int nstrip, int n, double *sc, void *sdata,
+                      double *u0, double *s, double *gu, double *Ju)
+{
+    double inv[64];
+    int unsolved = 0;
+    (void)nstrip;
+    for (int i = 0; i < n; ++i) inv[i] = 1.0 / sc[i];
+    for (int i = 0; i < n; ++i) u[i] *= inv[i];
+    for (int i = 0; i < n; ++i) atol[i] *= inv[i];
+    for (int i = 0; i < n; ++i) u0[i] = u[i];
+    for (int i = 0; i < n; ++i) s[i] = 0.0;
+    for (int i = 0; i < n; ++i) gu[i] = 0.0;
+    for (int i = 0; i < n * n; ++i) Ju[i] = 0.0;
+    for (int sweep = 0; sweep < 10; ++sweep) {
+        if (be_resid(f, u, u0, gu, dt, n, sc, inv, sdata) != 0 ||
+            be_jac(J, u, Ju, dt, n, sc, inv, sdata) != 0) {
+            for (int i = 0; i < n; ++i) u[i] *= sc[i];
+            for (int i = 0; i < n; ++i) atol[i] *= sc[i];
+            return 1;
+        }
+        unsolved = 0;
+        orc_gauss_elim(Ju, s, gu, n);
+        for (int i = 0; i < n; ++i) u[i] -= 1.0 * s[i];
+        for (int i = 0; i < n; ++i) {
+            if (fabs(s[i]) > (atol[i] + rtol[i] * fabs(u[i]))) {
+                unsolved = 1;
+                break;
+            }
+            if (u[i] != u[i]) {
+                for (int k = 0; k < n; ++k) u[k] *= sc[k];
+                for (int k = 0; k < n; ++k) atol[k] *= sc[k];
+                return 1;
+            }
+        }
+        if (!unsolved) break;
+    }
+    for (int i = 0; i < n; ++i) u[i] *= sc[i];
+    for (int i = 0; i < n; ++i) atol[i] *= sc[i];
+    return unsolved ? 1 : 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Substep driver (rates_and_rate_tables.C.template:149-274), one cell        */
+/* ------------------------------------------------------------------------ */
+typedef struct { int status, nsub, nrej; double dt; } cell_stats;
+
+static double evolve_cell(orc_be_f be, orc_prim_data *d, double dtf, double dt, double z,
+                          double *in, double *rtol, double *atol, int input_is_number,
+                          cell_stats *st)
+{
+    const int N = ORC_PRIM_N;
+    double scale[ORC_PRIM_N], prev[ORC_PRIM_N], u0[ORC_PRIM_N], s[ORC_PRIM_N],
+           gu[ORC_PRIM_N], Ju[ORC_PRIM_N * ORC_PRIM_N];
+    if (!input_is_number) {
+        for (int k = 0; k < N; ++k) {
+            if (k == GE) continue;
+            in[k] /= W[k] * 1.67e-24;
+            atol[k] /= W[k] * 1.67e-24;
+        }
+    }
+    /* ensure_electron_consistency (:604-630) for one cell */
+    double total_e = 0.0;
+    for (int k = 0; k < N; ++k)
+        if (k != GE && k != DE) total_e += in[k] * FREE_E[k];
+    in[DE] = total_e;
+
+    if (dt < 0) dt = dtf / 1e5;
+    d->current_z = z;
+    int siter = 0;
+    double ttot = 0;
+    for (int k = 0; k < N; ++k) scale[k] = in[k];
+    for (int k = 0; k < N; ++k) prev[k] = in[k];
+    st->status = 0;
+    st->nsub = 0;
+    st->nrej = 0;
+    while (ttot < dtf) {
+        int rv = be((orc_rhs_f)orc_prim_rhs, (orc_jac_f)orc_prim_jac, in, dt, rtol, atol,
+                    1, N, scale, (void *)d, u0, s, gu, Ju);
+        for (int k = 0; k < N; ++k) {
+            if (in[k] < 0) { rv = 1; break; }
+        }
+        if (rv == 0) {
+            if (siter == 50000) { st->status = 2; break; }
+            siter++;
+            ttot += dt;
+            dt = (dt * 1.1 < dtf - ttot) ? dt * 1.1 : dtf - ttot;
+            for (int k = 0; k < N; ++k) prev[k] = in[k];
+            for (int k = 0; k < N; ++k)
+                if (in[k] < 1e-25) in[k] = 1e-25;
+        } else {
+            st->nrej++;
+            dt /= 2.0;
+            for (int k = 0; k < N; ++k) in[k] = prev[k];
+            if (dt / dtf < 1e-15) { st->status = 1; break; }
+        }
+    }
+    st->nsub = siter;
+    st->dt = dt;
+    if (!input_is_number) {
+        for (int k = 0; k < N; ++k) {
+            if (k == GE) continue;
+            in[k] *= W[k] * 1.67e-24;
+            atol[k] *= W[k] * 1.67e-24;
+        }
+    }
+    return ttot;
+}
+
+double orc_prim_evolve(double dtf, double *dt, double z, double *input,
+                       double *rtol, double *atol, long long dims,
+                       const double *tables, int nT, double T0, double T1,
+                       int input_is_number, int *status, int *nsub, int *nrej,
+                       int *nsweep, double *dt_out, double *T_out,
+                       orc_be_f be, int nthreads, int flags)
+{
+    const double dt_in = *dt;
+    double tmin = 1e300, dtmin = dt_in;
+    long long imin = -1;
+    if (be == NULL) be = orc_BE_chem_solve;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+#endif
+    for (long long c = 0; c < dims; ++c) {
+        orc_prim_data d;
+        cell_stats st;
+        orc_prim_setup(&d, tables, nT, T0, T1);
+        d.flags = flags;
+        double t = evolve_cell(be, &d, dtf, dt_in, z, input + c * ORC_PRIM_N,
+                               rtol + c * ORC_PRIM_N, atol + c * ORC_PRIM_N,
+                               input_is_number, &st);
+        if (status) status[c] = st.status;
+        if (nsub) nsub[c] = st.nsub;
+        if (nrej) nrej[c] = st.nrej;
+        if (nsweep) nsweep[c] = (int)d.rhs_calls;
+        if (dt_out) dt_out[c] = st.dt;
+        if (T_out) T_out[c] = d.T;
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+        {
+            if (t < tmin || (t == tmin && c < imin)) {
+                tmin = t;
+                dtmin = st.dt;
+                imin = c;
+            }
+        }
+    }
+    (void)nthreads;
+    if (dims > 0) *dt = dtmin;
+    return dims > 0 ? tmin : 0.0;
+}
+
+int orc_prim_eval(const double *u, long long dims, const double *tables, int nT,
+                  double T0, double T1, double z, double *f, double *J,
+                  double *T, double *rates, int flags)
+{
+    int nneg = 0;
+    for (long long c = 0; c < dims; ++c) {
+        orc_prim_data d;
+        double uc[ORC_PRIM_N];
+        orc_prim_setup(&d, tables, nT, T0, T1);
+        d.flags = flags;
+        d.current_z = z;
+        memcpy(uc, u + c * ORC_PRIM_N, sizeof(uc));
+        int rv = orc_prim_rhs(uc, f + c * ORC_PRIM_N, 1, ORC_PRIM_N, &d);
+        if (rv) nneg++;
+        else if (J) orc_prim_jac(uc, J + c * ORC_PRIM_N * ORC_PRIM_N, 1, ORC_PRIM_N, &d);
+        if (T) T[c] = d.T;
+        if (rates) memcpy(rates + c * ORC_PRIM_NTAB, d.rs, sizeof(d.rs));
+    }
+    return nneg;
+}

@@ -1,0 +1,97 @@
+/*
+ * CPU FP64 oracle for the Dengo backward-Euler chemistry solver.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library restates the reference algorithm on
+ * the CPU so that the GPU path can be checked against it; it is imported only
+ * by tests/, by the smoke check and by bench.py's CPU-baseline leg.  Nothing
+ * in the product (dengo_amd/) links, loads or calls it.
+ *
+ * Semantics: one dengo_evolve call per cell (nstrip = 1), following
+ *   dengo/templates/rates_and_rate_tables.C.template:149-274 (substep driver),
+ *   :309-350 (temperature), :362-455 (interpolation), :460-522 (RHS),
+ *   :527-599 (Jacobian), :604-630 (electron consistency),
+ *   dengo/solvers/BE_chem_solve.C:79-382 (scaled Newton + Gauss_Elim).
+ */
+#ifndef DENGO_ORACLE_H
+#define DENGO_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int (*orc_rhs_f)(double *, double *, int, int, void *);
+typedef int (*orc_jac_f)(double *, double *, int, int, void *);
+/* Same signature as BE_chem_solve (dengo/solvers/BE_chem_solve.C:79-83). */
+typedef int (*orc_be_f)(orc_rhs_f, orc_jac_f, double *u, double dt, double *rtol,
+                        double *atol, int nstrip, int nchem, double *scaling,
+                        void *sdata, double *u0, double *s, double *gu, double *Ju);
+
+/* Primordial network: 10 unknowns, 45 tables (order of the emitted struct). */
+#define ORC_PRIM_N 10
+#define ORC_PRIM_NTAB 45
+
+/* Table values of the primordial network on temperatures T[0..n), written
+ * as out[t*n + i]; restates dengo/primordial_rates.py:47-293 and
+ * dengo/primordial_cooling.py:34-316 with libm. */
+void orc_primordial_tables(const double *T, int n, double *out);
+
+/* The scaled Newton solver, restated (BE_chem_solve.C:79-321). */
+int orc_BE_chem_solve(orc_rhs_f f, orc_jac_f J, double *u, double dt, double *rtol,
+                      double *atol, int nstrip, int nchem, double *scaling,
+                      void *sdata, double *u0, double *s, double *gu, double *Ju);
+/* Gauss_Elim, restated (BE_chem_solve.C:329-382). */
+int orc_gauss_elim(double *A, double *x, double *b, int n);
+
+/* Per-call state handed to the RHS/Jacobian callbacks (nstrip must be 1). */
+typedef struct orc_prim_data {
+    double bounds[2];
+    int nbins;
+    double dbin, idbin, lb;
+    const double *tables;   /* [45][nT] */
+    int nT;
+    double current_z;
+    /* last temperature / interpolation state (from the last RHS call) */
+    double T, logT, invT, dTs_ge, Tdef, dT;
+    int bin_id;
+    double rs[ORC_PRIM_NTAB];
+    long long rhs_calls;    /* one per Newton sweep */
+    int flags;              /* ORC_FLAG_* */
+} orc_prim_data;
+
+/* Take ln T with orc_plog instead of libm's log. */
+#define ORC_FLAG_PORTABLE_LOG 1
+double orc_plog(double x);
+
+void orc_prim_setup(orc_prim_data *d, const double *tables, int nT, double T0, double T1);
+int orc_prim_rhs(double *input, double *rhs, int nstrip, int nchem, void *sdata);
+int orc_prim_jac(double *input, double *J, int nstrip, int nchem, void *sdata);
+void orc_prim_temperature(orc_prim_data *d, const double *u);
+void orc_prim_interpolate(orc_prim_data *d);
+
+/* Whole-step driver over `dims` independent cells.  `input`, `rtol`, `atol`
+ * are [dims][10] as in dengo_evolve; input and atol are updated in place.
+ * dt: in = initial substep (<0 selects dtf/1e5), out = final dt of the cell
+ * with the smallest elapsed time.  Returns the smallest elapsed time.
+ * Optional per-cell outputs (may be NULL): status (0 reached dtf, 1 gave up
+ * at dt/dtf < 1e-15, 2 hit the 50000-substep cap), accepted and rejected
+ * substeps, Newton sweeps, final dt, final temperature.
+ * be: NULL selects orc_BE_chem_solve.  nthreads <= 0 uses OpenMP's default.
+ * flags: ORC_FLAG_* (0 = libm log, as the reference). */
+double orc_prim_evolve(double dtf, double *dt, double z, double *input,
+                       double *rtol, double *atol, long long dims,
+                       const double *tables, int nT, double T0, double T1,
+                       int input_is_number, int *status, int *nsub, int *nrej,
+                       int *nsweep, double *dt_out, double *T_out,
+                       orc_be_f be, int nthreads, int flags);
+
+/* Evaluate RHS (f[dims][10]), Jacobian (J[dims][10][10], column-major per
+ * cell) and temperature for states u (number densities, ge) at redshift z.
+ * Returns the number of cells whose RHS reported a negative value. */
+int orc_prim_eval(const double *u, long long dims, const double *tables, int nT,
+                  double T0, double T1, double z, double *f, double *J,
+                  double *T, double *rates, int flags);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
