@@ -1,0 +1,190 @@
+"""Benchmark: cell-solves/s of the primordial backward-Euler solve on one MI355X.
+
+Workload: 128^3 = 2,097,152 primordial cells (10 unknowns, 45 tables on 1024
+log-T bins over 1-1e9 K), seeded synthetic inputs (density 1e-28..1e-18 g/cm^3,
+T 10..1e8 K, He mass fraction 0.24, ionised fraction U(0,1), H2 fraction
+1e-8..1e-2), one step dtf = 1e12 s with adaptive substeps, rtol 1e-9,
+atol 1e-9 x initial value.  A "step" is one full dengo_evolve over all cells,
+inputs resident in HBM.
+
+Variants timed (all on the same inputs):
+  fp64_persistent  reference-faithful FP64, persistent lane-per-cell schedule
+                   (its per-cell sweep counts equal the CPU oracle's and give
+                   the algorithmic work for the roofline)
+  fp64_percell     the plain one-cell-per-lane FP64 kernel (drop-in baseline)
+  headline         the variant `value` reports (--variant)
+The CPU baseline is the oracle (plain C restatement, OpenMP over cells) on the
+first --cpu-cells cells of the same inputs, on the host's cores.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Algorithmic FP64 operations per Newton sweep of the primordial network
+# (divisions and compares count as one): RHS 465 (T 36, ln T 29,
+# interpolation 144, mdensity 9, fluxes 46, accumulation 132, cooling 69),
+# residual/scaling 80, Jacobian 447, Jacobian scaling 310, Gauss_Elim 805
+# (45 multipliers, 570 update, 90 forward, 100 back-substitution), update and
+# convergence test 30.
+F_SWEEP = 2137
+PEAK_FP64 = 78.6e12      # MI355X FP64 (vector and matrix) TFLOP/s
+PEAK_FP32 = 157.3e12     # MI355X FP32 vector
+
+
+def log(msg):
+    sys.stderr.write("[bench] %s\n" % msg)
+    sys.stderr.flush()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cells", type=int, default=128 ** 3)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--variant", default="fp64_persistent",
+                    choices=["fp64_persistent", "fp64_percell", "mixed_persistent"])
+    ap.add_argument("--no-percell", action="store_true", help="skip the plain per-cell kernel")
+    ap.add_argument("--cpu-cells", type=int, default=4096)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: this build has no multi-GPU path "
+              "(cells would be split by domain across GPUs; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from dengo_amd import runtime
+    from dengo_amd.ics import primordial_ics, tolerances
+    from dengo_amd.networks import primordial_tables
+
+    dev = torch.device("cuda:0")
+    names, tab = primordial_tables()
+    solver = runtime.PrimordialSolver(tab)
+    n = args.cells
+    log("generating %d cells" % n)
+    inp, _, _ = primordial_ics(n, seed=args.seed)
+    rtol, atol = tolerances(inp)
+    d_inp = torch.from_numpy(inp).to(dev)
+    d_rtol = torch.from_numpy(rtol).to(dev)
+    d_atol = torch.from_numpy(atol).to(dev)
+    stream = torch.cuda.current_stream()
+
+    def run(variant, x, at, outputs=None):
+        prec = runtime.MIXED if variant.startswith("mixed") else runtime.FP64
+        sched = runtime.PER_CELL if variant.endswith("percell") else runtime.PERSISTENT
+        tmin, _ = solver.evolve_device(x, d_rtol, at, 1e12, outputs=outputs, stream=stream,
+                                       precision=prec, schedule=sched)
+        return tmin, runtime.load().dengo_last_kernel_ms()
+
+    # reference-faithful FP64 run: canonical sweep counts and FP64 result
+    outs = {"status": torch.zeros(n, dtype=torch.int32, device=dev),
+            "nsweep": torch.zeros(n, dtype=torch.int32, device=dev),
+            "nsub": torch.zeros(n, dtype=torch.int32, device=dev)}
+    x64, a64 = d_inp.clone(), d_atol.clone()
+    log("fp64_persistent reference run")
+    t0 = time.perf_counter()
+    _, ms64 = run("fp64_persistent", x64, a64, outs)
+    wall64 = time.perf_counter() - t0
+    sweeps = outs["nsweep"].to(torch.int64).sum().item()
+    status = outs["status"].cpu().numpy()
+    flops = float(sweeps) * F_SWEEP
+    log("fp64_persistent: %.1f ms kernel, %d sweeps, capped %d" % (ms64, sweeps, (status == 2).sum()))
+
+    ms_percell = None
+    if not args.no_percell and args.variant != "fp64_percell":
+        x1, a1 = d_inp.clone(), d_atol.clone()
+        log("fp64_percell run")
+        _, ms_percell = run("fp64_percell", x1, a1)
+        same = bool(torch.equal(x1, x64))
+        log("fp64_percell: %.1f ms kernel, bitwise equal to persistent: %s" % (ms_percell, same))
+        del x1, a1
+
+    # headline: warmup + timed steps on fresh copies of the resident inputs
+    W, K = args.warmup, args.steps
+    xs = [d_inp.clone() for _ in range(W + K)]
+    ats = [d_atol.clone() for _ in range(W + K)]
+    torch.cuda.synchronize()
+    for i in range(W):
+        run(args.variant, xs[i], ats[i])
+    torch.cuda.synchronize()
+    kms = []
+    t0 = time.perf_counter()
+    for i in range(W, W + K):
+        _, ms = run(args.variant, xs[i], ats[i])
+        kms.append(ms)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = 1e3 * wall / K
+    value = n / (wall / K)
+    kernel_ms = float(np.mean(kms))
+    log("%s: %.1f ms/step (kernel %.1f ms)" % (args.variant, ms_step, kernel_ms))
+
+    # parity of the headline result against the reference-faithful FP64 run
+    xh = xs[-1]
+    equal = bool(torch.equal(xh, x64))
+    done = torch.from_numpy(status == 0).to(dev)
+    rel = ((xh - x64).abs() / x64.abs().clamp_min(1e-300))[done]
+    parity = {"vs": "fp64_persistent (bit-identical to the CPU oracle)",
+              "bitwise_equal": equal,
+              "max_rel_err_completed_cells": float(rel.max().item()) if rel.numel() else 0.0}
+
+    # roofline: algorithmic FLOPs from the faithful sweep counts
+    if args.variant.startswith("mixed"):
+        peak = PEAK_FP64  # refined when the FP32 share is known
+    else:
+        peak = PEAK_FP64
+    achieved = flops / (kernel_ms * 1e-3)
+    roof = {"bound": "mfma", "compute_unit": "FP64 vector ALU (MI355X FP64 vector and "
+            "matrix peaks are both 78.6 TFLOP/s)", "achieved": achieved / 1e12,
+            "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
+            "traffic": None, "flops_per_launch": flops, "F_sweep": F_SWEEP,
+            "sweeps_per_launch": sweeps}
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        m = min(args.cpu_cells, n)
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+        log("cpu baseline: oracle on %d cells, %d threads" % (m, threads))
+        t0 = time.perf_counter()
+        oracle.prim_evolve(inp[:m], rtol[:m], atol[:m], tab, 1e12, nthreads=threads)
+        el = time.perf_counter() - t0
+        cpu = {"value": m / el, "unit": "cell-solves/s", "cores": threads, "kind": "port",
+               "sample": "first %d of the %d cells (same seeded inputs), CPU FP64 oracle, "
+                         "OpenMP dynamic schedule, %.1f s" % (m, n, el)}
+
+    out = {
+        "metric": "cell-solves/s (primordial BE solve, dtf=1e12 s)",
+        "value": value, "unit": "cell-solves/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if not args.variant.startswith("mixed") else "f64+f32",
+        "data": "synthetic (seeded, config distributions)",
+        "config": {"workload": "primordial 10-unknown network, 128^3 = %d cells, one "
+                               "dtf=1e12 s step" % n, "variant": args.variant,
+                   "cells": n, "seed": args.seed},
+        "fp64_persistent_kernel_ms": ms64,
+        "fp64_percell_kernel_ms": ms_percell,
+        "speedup_vs_fp64_percell": (ms_percell / kernel_ms) if ms_percell else None,
+        "speedup_vs_fp64_persistent": ms64 / kernel_ms,
+        "capped_cells": int((status == 2).sum()),
+        "parity": parity,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -79,6 +79,14 @@ def load(path=LIB_PATH):
     if _lib is None:
         if not os.path.exists(path):
             raise RuntimeError("HIP library %s not built: run dengo_amd/build.py" % path)
+        # One HIP runtime per process: torch ships its own libamdhip64 (same
+        # soname).  Loading torch first makes this library bind to that copy;
+        # loading ours first would make torch load a second runtime that then
+        # finds no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = ctypes.CDLL(path)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)
@@ -160,8 +168,8 @@ class PrimordialSolver(object):
             pass
 
     def evolve(self, inp, rtol, atol, dtf, dt=-1.0, z=-1.0, **opt):
-        """dengo_evolve semantics on host copies; returns a dict like the
-        oracle's (state, atol, ttot, dt, per-cell status/nsub/nrej/nsweep/dt/T)."""
+        """dengo_evolve semantics on host copies; returns a dict with the final
+        state, atol, ttot, dt and per-cell status/nsub/nrej/nsweep/dt/T."""
         lib = load()
         out = np.array(inp, dtype=np.float64, copy=True, order="C")
         at = np.array(atol, dtype=np.float64, copy=True, order="C")

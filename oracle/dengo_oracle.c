@@ -586,6 +586,11 @@ int orc_BE_chem_solve(orc_rhs_f f, orc_jac_f J, double *u, double dt, double *rt
 /* ------------------------------------------------------------------------ */
 typedef struct { int status, nsub, nrej; double dt; } cell_stats;
 
+/* Accepted-substep cap (rates_and_rate_tables.C.template:204 hard-codes
+ * 50000); tests lower it to reach the cap quickly. */
+static int g_max_substeps = 50000;
+void orc_set_max_substeps(int n) { g_max_substeps = n > 0 ? n : 50000; }
+
 static double evolve_cell(orc_be_f be, orc_prim_data *d, double dtf, double dt, double z,
                           double *in, double *rtol, double *atol, int input_is_number,
                           cell_stats *st)
@@ -622,7 +627,7 @@ static double evolve_cell(orc_be_f be, orc_prim_data *d, double dtf, double dt, 
             if (in[k] < 0) { rv = 1; break; }
         }
         if (rv == 0) {
-            if (siter == 50000) { st->status = 2; break; }
+            if (siter == g_max_substeps) { st->status = 2; break; }
             siter++;
             ttot += dt;
             dt = (dt * 1.1 < dtf - ttot) ? dt * 1.1 : dtf - ttot;

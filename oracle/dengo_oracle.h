@@ -58,6 +58,9 @@ typedef struct orc_prim_data {
     int flags;              /* ORC_FLAG_* */
 } orc_prim_data;
 
+/* Accepted-substep cap (default 50000, as the reference). */
+void orc_set_max_substeps(int n);
+
 /* Take ln T with orc_plog instead of libm's log. */
 #define ORC_FLAG_PORTABLE_LOG 1
 double orc_plog(double x);

@@ -40,6 +40,8 @@ def load():
         lib.orc_plog.argtypes = [ctypes.c_double]
         lib.orc_primordial_tables.restype = None
         lib.orc_primordial_tables.argtypes = [_dp, ctypes.c_int, _dp]
+        lib.orc_set_max_substeps.restype = None
+        lib.orc_set_max_substeps.argtypes = [ctypes.c_int]
         lib.orc_gauss_elim.restype = ctypes.c_int
         lib.orc_gauss_elim.argtypes = [_dp, _dp, _dp, ctypes.c_int]
         _LIB = lib
@@ -93,9 +95,11 @@ def prim_eval(u, tables, T_bounds=(1.0, 1e9), z=-1.0, flags=0):
 
 
 def prim_evolve(inp, rtol, atol, tables, dtf, dt=-1.0, z=-1.0, T_bounds=(1.0, 1e9),
-                input_is_number=False, use_ref=False, nthreads=0, flags=0):
+                input_is_number=False, use_ref=False, nthreads=0, flags=0,
+                max_substeps=50000):
     """Run the oracle on copies of the inputs.  Returns a dict with the final
     state, atol, ttot, dt and per-cell counters."""
+    load().orc_set_max_substeps(max_substeps)
     out = np.array(inp, dtype=np.float64, copy=True, order="C")
     atol_o = np.array(atol, dtype=np.float64, copy=True, order="C")
     rtol_c = np.array(rtol, dtype=np.float64, copy=True, order="C")

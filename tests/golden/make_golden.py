@@ -1,0 +1,39 @@
+"""Regenerate the primordial golden fixtures from the CPU oracle.
+
+Writes tests/golden/primordial_cells.npz: 96 seeded cells (config-1
+distributions, seed 11) with inputs, tolerances and the oracle's outputs
+(final state, atol, elapsed time, per-cell counters) for ln T taken with libm
+("libm_*") and with the portable log the device code uses ("plog_*").
+dtf = 1e12 s, dt = -1, z = -1, as the emitted main runs the solver.
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "oracle")]
+
+import oracle  # noqa: E402
+from dengo_amd.ics import primordial_ics, tolerances  # noqa: E402
+from dengo_amd.networks import primordial_tables  # noqa: E402
+
+
+def main():
+    _, tab = primordial_tables()
+    inp, _, _ = primordial_ics(96, seed=11)
+    rtol, atol = tolerances(inp)
+    out = dict(input=inp, rtol=rtol, atol=atol, dtf=1e12, seed=11)
+    for tag, flags in (("libm", 0), ("plog", oracle.PORTABLE_LOG)):
+        r = oracle.prim_evolve(inp, rtol, atol, tab, 1e12, flags=flags)
+        for k in ("state", "atol", "status", "nsub", "nrej", "nsweep", "dt_cell", "T"):
+            out["%s_%s" % (tag, k)] = r[k]
+        out["%s_ttot" % tag] = r["ttot"]
+        out["%s_dt" % tag] = r["dt"]
+    np.savez_compressed(os.path.join(HERE, "primordial_cells.npz"), **out)
+    print("wrote primordial_cells.npz; capped cells:", int((out["libm_status"] == 2).sum()))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,48 @@
+"""The C-ABI library loads and exports every function include/*.h declares
+(no GPU work is done here)."""
+import ctypes
+import glob
+import os
+import re
+
+from dengo_amd import runtime
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"static inline[^{]*\{[^}]*\}", "", text)
+        for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text):
+            name = m.group(1)
+            line_start = text.rfind("\n", 0, m.start()) + 1
+            line = text[line_start:m.start()]
+            if line.lstrip().startswith("#") or name in ("X", "sizeof", "if", "return"):
+                continue
+            if re.search(r"(void|int|double|long long|char|_data)\s*\**\s*$", line):
+                names.add(name)
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = runtime.load()
+    decl = declared_functions()
+    assert "dengo_evolve_primordial" in decl and "primordial_setup_data" in decl
+    missing = [n for n in sorted(decl) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_bindings_cover_declarations():
+    assert declared_functions() <= set(runtime.SIGNATURES), \
+        declared_functions() - set(runtime.SIGNATURES)
+
+
+def test_no_oracle_in_product():
+    # the product never loads the CPU oracle
+    for path in glob.glob(os.path.join(ROOT, "dengo_amd", "**", "*.*"), recursive=True):
+        if path.endswith((".py", ".hip", ".h", ".cpp")):
+            text = open(path).read()
+            assert "import oracle" not in text and "liboracle" not in text, path

@@ -1,0 +1,236 @@
+"""Pins on the CPU oracle (no GPU).
+
+Each test is chosen so that a plausible slip in the restatement (a dropped
+term, a wrong sign or index, a transposed operand) fails it:
+
+* the reference's own BE_chem_solve.C (compiled unmodified into oracle/_ref)
+  driven by the oracle's RHS/Jacobian reproduces the oracle bit for bit;
+* Gauss_Elim agrees with LAPACK (numpy.linalg.solve), including pivot-forcing
+  matrices;
+* the analytic Jacobian matches finite differences of the RHS at frozen T;
+* the RHS conserves H and He nuclei for every state and conserves charge except
+  for k12's written-as-is 2 HII product;
+* f = 0 reproduces the closed-form geometric substep count;
+* a single-reaction network reproduces the closed-form BE step;
+* the C table restatement matches the numpy front end and known values.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from dengo_amd.ics import primordial_ics, tolerances
+
+H2I, H2II, HI, HII, HM, HeI, HeII, HeIII, DE, GE = range(10)
+
+
+def test_oracle_matches_reference_be_chem_solve(prim_tables):
+    if oracle.load_ref() is None:
+        pytest.skip("reference BE_chem_solve.C not built (reference tree absent)")
+    _, tab = prim_tables
+    inp, _, _ = primordial_ics(40, seed=5)
+    rtol, atol = tolerances(inp)
+    a = oracle.prim_evolve(inp, rtol, atol, tab, 1e12)
+    b = oracle.prim_evolve(inp, rtol, atol, tab, 1e12, use_ref=True)
+    assert (a["status"] == 2).any(), "fixture should include capped cells"
+    for k in ("state", "atol", "status", "nsub", "nrej", "nsweep", "dt_cell", "T"):
+        assert np.array_equal(a[k], b[k]), k
+    assert a["ttot"] == b["ttot"] and a["dt"] == b["dt"]
+
+
+def test_golden_fixture_reproduces(prim_tables):
+    _, tab = prim_tables
+    g = np.load("tests/golden/primordial_cells.npz")
+    for tag, flags in (("libm", 0), ("plog", oracle.PORTABLE_LOG)):
+        r = oracle.prim_evolve(g["input"], g["rtol"], g["atol"], tab, float(g["dtf"]),
+                               flags=flags)
+        for k in ("state", "atol", "status", "nsub", "nrej", "nsweep"):
+            assert np.array_equal(r[k], g["%s_%s" % (tag, k)]), (tag, k)
+
+
+def test_gauss_elim_vs_lapack():
+    rng = np.random.default_rng(0)
+    for n in (1, 2, 3, 10, 17):
+        for trial in range(20):
+            A = rng.normal(size=(n, n))
+            if trial % 2:
+                # force pivoting: tiny diagonal, large off-diagonal
+                A[np.arange(n), np.arange(n)] *= 1e-8
+            b = rng.normal(size=n)
+            x = oracle.gauss_elim(A.copy(), b)
+            xr = np.linalg.solve(A, b)
+            assert np.allclose(x, xr, rtol=1e-9, atol=1e-12 * np.abs(xr).max())
+
+
+def test_gauss_elim_pivot_rule():
+    # Column 0 has the same |a| in rows 1 and 2: the first strictly larger
+    # entry wins, so row 1 becomes the pivot.  The answer is unique, so check
+    # the solution and that zero pivots in column 0 of row 0 are handled.
+    A = np.array([[0.0, 1.0, 2.0], [3.0, 1.0, 0.0], [-3.0, 2.0, 1.0]])
+    b = np.array([1.0, 2.0, 3.0])
+    assert np.allclose(oracle.gauss_elim(A.copy(), b), np.linalg.solve(A, b), rtol=1e-14)
+
+
+def _keep_T(u, T):
+    """Reset ge so the solver's temperature stays T (freezes the rates)."""
+    from dengo_amd.ics import primordial_temperature
+    v = u.copy()
+    v[..., GE] = 1.0
+    _, density, gsum = primordial_temperature(v)
+    v[..., GE] = T * 1.3806504e-16 * gsum / (density * 1.67e-24)
+    return v
+
+
+def test_jacobian_matches_finite_differences(prim_tables):
+    _, tab = prim_tables
+    _, n, T = primordial_ics(64, seed=21, T_range=(20.0, 5e7))
+    f0, J, T0, _, neg = oracle.prim_eval(n, tab)
+    assert neg == 0
+
+    def mdens(u):
+        # the energy row is (cooling sum)/mdensity with mdensity = mh * sum of
+        # species other than de, ge; the reference's Jacobian differentiates the
+        # cooling sum only, so compare d(cooling)/dn = d(f_ge * mdensity)/dn
+        return 1.67e-24 * u[:, :DE].sum(axis=1)
+
+    def cool_form(f, u):
+        g = f.copy()
+        g[:, GE] = f[:, GE] * mdens(u)
+        return g
+
+    for col in range(GE):
+        h = 1e-6 * n[:, col]
+        up = n.copy()
+        dn = n.copy()
+        up[:, col] += h
+        dn[:, col] -= h
+        upT, dnT = _keep_T(up, T0), _keep_T(dn, T0)
+        fu = cool_form(oracle.prim_eval(upT, tab)[0], upT)
+        fd = cool_form(oracle.prim_eval(dnT, tab)[0], dnT)
+        fdj = (fu - fd) / (2 * h[:, None])
+        an = J[:, col, :].copy()
+        an[:, GE] *= mdens(n)
+        f0 = cool_form(oracle.prim_eval(n, tab)[0], n)
+        scale = np.abs(f0) / np.maximum(n[:, col:col + 1], 1e-300) + np.abs(an)
+        err = np.abs(fdj - an) / np.maximum(scale, 1e-300)
+        assert err.max() < 1e-5, (col, err.max())
+    # the energy column is identically zero (reference quirk)
+    assert np.all(J[:, GE, :] == 0.0)
+
+
+def test_rhs_conservation(prim_tables):
+    _, tab = prim_tables
+    _, n, _ = primordial_ics(200, seed=4)
+    f, _, _, rates, _ = oracle.prim_eval(n, tab)
+    mag = np.abs(f[:, :GE]).max(axis=1)
+    h = f[:, HI] + f[:, HII] + f[:, HM] + 2 * f[:, H2I] + 2 * f[:, H2II]
+    he = f[:, HeI] + f[:, HeII] + f[:, HeIII]
+    assert np.all(np.abs(h) <= 1e-12 * mag)
+    assert np.all(np.abs(he) <= 1e-12 * mag)
+    charge = f[:, HII] + f[:, HeII] + 2 * f[:, HeIII] + f[:, H2II] - f[:, HM] - f[:, DE]
+    k12 = rates[:, 11] * n[:, H2I] * n[:, DE]   # H2I + e -> 2 HII + e
+    assert np.allclose(charge, 2 * k12, rtol=1e-9, atol=1e-12 * mag.max())
+
+
+def test_zero_rhs_substep_count(prim_tables):
+    names, tab = prim_tables
+    zero = np.zeros_like(tab)
+    # the H2 cooling term divides by galdl = sum of ga* tables: keep those at 1
+    # (gphdl = 0 makes the term exactly 0) so that f is identically zero
+    for i, nm in enumerate(names):
+        if nm.startswith("gloverabel08_ga"):
+            zero[i] = 1.0
+    inp, _, _ = primordial_ics(8, seed=1)
+    rtol, atol = tolerances(inp)
+    dtf = 1e12
+    r = oracle.prim_evolve(inp, rtol, atol, zero, dtf)
+    # closed form: smallest n with dt0 * (1.1^n - 1)/0.1 >= dtf, dt0 = dtf/1e5
+    n_closed = math.ceil(math.log(1 + 0.1 * 1e5) / math.log(1.1))
+    assert np.all(r["status"] == 0)
+    assert np.all(np.abs(r["nsub"] - n_closed) <= 1)
+    assert np.all(r["nsweep"] == r["nsub"])        # one sweep per step when f = 0
+    assert np.all(r["nrej"] == 0)
+    assert r["ttot"] >= dtf
+
+
+def test_single_reaction_closed_form(prim_tables):
+    # Only k10 (H2II + HI -> H2I + HII, constant 6e-10): one BE step of
+    # length dtf has the closed form x1 = positive root of
+    # k dt x^2 + (1 + k dt (y0 - x0)) x - x0 = 0  (x = H2II, y = HI).
+    names, tab = prim_tables
+    t = np.zeros_like(tab)
+    t[names.index("k10"), :] = 6.0e-10
+    for i, nm in enumerate(names):
+        if nm.startswith("gloverabel08_ga"):
+            t[i] = 1.0      # keeps the H2 cooling term at exactly 0 (see above)
+    _, n, _ = primordial_ics(16, seed=2)
+    n[:, H2II] = n[:, HI] * np.linspace(0.1, 0.9, 16)  # comparable abundances
+    rtol = np.full_like(n, 1e-12)
+    atol = n * 1e-14
+    k = 6.0e-10
+    dt = 0.3 / (k * n[:, HI].max())
+    out = []
+    for c in range(16):
+        r = oracle.prim_evolve(n[c:c + 1], rtol[c:c + 1], atol[c:c + 1], t, dt, dt=dt,
+                               input_is_number=True)
+        assert r["status"][0] == 0 and r["nsub"][0] == 1
+        out.append(r["state"][0])
+    out = np.array(out)
+    x0, y0 = n[:, H2II], n[:, HI]
+    a, b = k * dt, 1 + k * dt * (y0 - x0)
+    x1 = (-b + np.sqrt(b * b + 4 * a * x0)) / (2 * a)
+    assert np.allclose(out[:, H2II], x1, rtol=1e-9)
+    assert np.allclose(out[:, HI], x1 + (y0 - x0), rtol=1e-9)
+    assert np.allclose(out[:, H2I] - n[:, H2I], x0 - x1, rtol=1e-8)
+
+
+def test_tables_c_restatement_matches_front_end(prim_tables):
+    from dengo_amd.networks import primordial_network
+    names, tab = prim_tables
+    with np.errstate(all="ignore"):
+        T = primordial_network().T
+    c = oracle.primordial_tables(T)
+    assert c.shape == tab.shape
+    big = np.maximum(np.abs(tab), np.abs(c))
+    err = np.abs(tab - c) / np.where(big > 0, big, 1.0)
+    # numpy's and libm's exp/pow differ by an ulp; the worst case is k23 at
+    # 1e9 K where 1 - exp(-6000/T) ~ 6e-6 amplifies that ulp by ~2e5.
+    assert err.max() < 1e-10, names[int(np.argmax(err.max(axis=1)))]
+    assert np.median(err) < 1e-15
+
+
+def test_table_known_values(prim_tables):
+    names, tab = prim_tables
+    idx = {n: i for i, n in enumerate(names)}
+    from dengo_amd.networks import primordial_network
+    with np.errstate(all="ignore"):
+        net = primordial_network()
+    T, tev = net.T, net.tev
+    assert np.all(tab[idx["k10"]] == 6.0e-10)
+    assert np.all(tab[idx["compton_comp"]] == 5.65e-36)
+    assert np.allclose(tab[idx["k21"]], 2.8e-31 * T ** -0.6, rtol=1e-14)
+    assert np.allclose(tab[idx["k22"]], 7.7e-31 * T ** -0.464, rtol=1e-14)
+    # branch switch points
+    assert np.all(tab[idx["k03"]][tev <= 0.8] == 1e-30)
+    assert np.all(tab[idx["k03"]][tev > 0.8] != 1e-30)
+    assert np.all(tab[idx["k08"]][tev <= 0.1] == 1.43e-9)
+    assert np.all(tab[idx["k18"]][T <= 617] == 1e-8)
+    assert np.allclose(tab[idx["k02"]][tev < 0.8], 3.92e-13 / tev[tev < 0.8] ** 0.6353)
+    # derived tables are exact multiples of the rate they scale
+    for t, r, c in (("ciHI_ciHI", "k01", 2.18e-11), ("ciHeI_ciHeI", "k03", 3.94e-11),
+                    ("ciHeII_ciHeII", "k05", 8.72e-11),
+                    ("h2formation_h2mheat", "k22", 7.177e-12),
+                    ("h2formation_h2mcool", "k13", 7.177e-12)):
+        assert np.array_equal(tab[idx[t]], c * tab[idx[r]]), t
+    assert np.allclose(tab[idx["gammah_gammah"]], 8.5e-26 + T)
+
+
+def test_portable_log_accuracy():
+    rng = np.random.default_rng(3)
+    lib = oracle.load()
+    xs = np.exp(rng.uniform(-50, 50, 20000))
+    for x in xs:
+        p = lib.orc_plog(float(x))
+        assert abs(p - math.log(x)) <= 1.01 * math.ulp(math.log(x)) + 1e-300
+    assert lib.orc_plog(1.0) == 0.0
