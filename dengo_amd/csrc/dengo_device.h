@@ -16,6 +16,10 @@
 
 namespace dengo {
 
+// Value of a static_for index (a std::integral_constant, possibly captured by
+// reference in a nested lambda).
+#define CV(x) (std::remove_reference_t<decltype(x)>::value)
+
 // Compile-time loop: f(std::integral_constant<int, I>) for I in [B, E).
 template <int B, int E, class F>
 DENGO_HD void static_for(F &&f)
@@ -283,12 +287,12 @@ DENGO_D bool rhs(const double (&u)[N], double (&f)[N], double (&rs)[NTAB], doubl
 #pragma unroll
     for (int s = 0; s < N; ++s) f[s] = 0.0;
     static_for<0, NRXN>([&](auto r) {
-        constexpr Rxn R = RXN[decltype(r)::value];
+        constexpr Rxn R = RXN[CV(r)];
         double flux = rs[R.table];
-        static_for<0, R.nslot>([&](auto k) { flux = flux * u[R.slot[decltype(k)::value]]; });
+        static_for<0, R.nslot>([&](auto k) { flux = flux * u[R.slot[CV(k)]]; });
         static_for<0, GE>([&](auto s) {
-            constexpr int v = nu(decltype(r)::value, decltype(s)::value);
-            if constexpr (v != 0) f[decltype(s)::value] += (double)v * flux;
+            constexpr int v = nu(CV(r), CV(s));
+            if constexpr (v != 0) f[CV(s)] += (double)v * flux;
         });
     });
     f[GE] = cooling(rs, u, T, env.zp1, env.comp4) / md;
@@ -305,17 +309,17 @@ DENGO_D void jacobian(const double (&u)[N], const double (&rs)[NTAB], double T,
 #pragma unroll
     for (int k = 0; k < N * N; ++k) J[k] = 0.0;
     static_for<0, NRXN>([&](auto r) {
-        constexpr Rxn R = RXN[decltype(r)::value];
+        constexpr Rxn R = RXN[CV(r)];
         static_for<0, R.nslot>([&](auto j) {
             double part = rs[R.table];
             static_for<0, R.nslot>([&](auto k) {
-                if constexpr (decltype(k)::value != decltype(j)::value)
-                    part = part * u[R.slot[decltype(k)::value]];
+                if constexpr (CV(k) != CV(j))
+                    part = part * u[R.slot[CV(k)]];
             });
-            constexpr int col = R.slot[decltype(j)::value];
+            constexpr int col = R.slot[CV(j)];
             static_for<0, GE>([&](auto s) {
-                constexpr int v = nu(decltype(r)::value, decltype(s)::value);
-                if constexpr (v != 0) J[col * N + decltype(s)::value] += (double)v * part;
+                constexpr int v = nu(CV(r), CV(s));
+                if constexpr (v != 0) J[col * N + CV(s)] += (double)v * part;
             });
         });
     });

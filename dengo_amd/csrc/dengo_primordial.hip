@@ -1,10 +1,12 @@
 // Primordial-network kernels and the C ABI of the HIP backend.
 //
 // Kernels (one cell per lane, state in registers, tables in HBM/L2):
-//   k_prim_fp64<false>  one cell per lane, grid covers all cells
-//   k_prim_fp64<true>   persistent: resident lanes pull cells from a device
-//                       counter, wave-aggregated, one Newton sweep per loop
-//   k_prim_eval         batched RHS / Jacobian / temperature evaluation
+//   k_prim<false, MODE>  one cell per lane, grid covers all cells
+//   k_prim<true, MODE>   persistent: resident lanes pull cells from a device
+//                        counter, wave-aggregated, one Newton sweep per loop
+//   MODE: kFaithful (reference-faithful FP64), kFast64 (block-structured FP64
+//         solve), kMixed (FP32 LU of the equilibrated block, FP64 residual)
+//   k_prim_eval          batched RHS / Jacobian / temperature evaluation
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -16,6 +18,7 @@
 #include <vector>
 
 #include "be_driver.h"
+#include "be_fast.h"
 #include "dengo_hip.h"
 
 using namespace dengo;
@@ -25,15 +28,37 @@ using namespace dengo;
 // ---------------------------------------------------------------------------
 constexpr int kBlock = 64;
 
-template <bool PERSIST>
-__global__ void __launch_bounds__(kBlock) k_prim_fp64(KParams<PrimNet> p)
+enum { kFaithful = 0, kFast64 = 1, kMixed = 2 };
+
+template <int MODE>
+DENGO_D bool cell_start(Cell<PrimNet> &c, long long id, const KParams<PrimNet> &p)
+{
+    if constexpr (MODE == kFaithful) return start_or_finish(c, id, p);
+    else return start_or_finish_fast(c, id, p);
+}
+
+template <int MODE>
+DENGO_D bool cell_sweep(Cell<PrimNet> &c, const KParams<PrimNet> &p)
+{
+    if constexpr (MODE == kFaithful) {
+        const int code = sweep_fp64(c, p);
+        return code != SW_CONTINUE && end_be(c, code, p);
+    } else {
+        const int code = sweep_fast<PrimNet, typename std::conditional<MODE == kMixed, float,
+                                                                       double>::type>(c, p);
+        return code != SW_CONTINUE && end_be_fast(c, code, p);
+    }
+}
+
+template <bool PERSIST, int MODE>
+__global__ void __launch_bounds__(kBlock) k_prim(KParams<PrimNet> p)
 {
     Cell<PrimNet> c;
     bool active = false;
     bool exhausted = false;
     if (!PERSIST) {
         const long long id = (long long)blockIdx.x * kBlock + threadIdx.x;
-        if (id < p.ncells) active = start_or_finish(c, id, p);
+        if (id < p.ncells) active = cell_start<MODE>(c, id, p);
         exhausted = true;
     }
     const int lane = threadIdx.x & 63;
@@ -51,7 +76,7 @@ __global__ void __launch_bounds__(kBlock) k_prim_fp64(KParams<PrimNet> p)
                     const unsigned long long rank =
                         __popcll(mask & ((lane ? (~0ull >> (64 - lane)) : 0ull)));
                     const unsigned long long id = base + rank;
-                    if (id < (unsigned long long)p.ncells) active = start_or_finish(c, (long long)id, p);
+                    if (id < (unsigned long long)p.ncells) active = cell_start<MODE>(c, (long long)id, p);
                     else exhausted = true;
                 }
             }
@@ -59,14 +84,9 @@ __global__ void __launch_bounds__(kBlock) k_prim_fp64(KParams<PrimNet> p)
         } else {
             if (!__any(active)) break;
         }
-        if (active) {
-            const int code = sweep_fp64(c, p);
-            if (code != SW_CONTINUE) {
-                if (end_be(c, code, p)) {
-                    finish_cell(c, p);
-                    active = false;
-                }
-            }
+        if (active && cell_sweep<MODE>(c, p)) {
+            finish_cell(c, p);
+            active = false;
         }
     }
 }
@@ -329,7 +349,6 @@ int launch_evolve(primordial_data *d, DevState *s, const dengo_options &o, doubl
                   double z, double *d_input, const double *d_rtol, double *d_atol, long long dims,
                   const dengo_cell_outputs &out, hipStream_t stream)
 {
-    if (o.precision != DENGO_FP64) return fail("precision %d not available", o.precision);
     KParams<PrimNet> p;
     p.input = d_input;
     p.rtol = d_rtol;
@@ -350,11 +369,18 @@ int launch_evolve(primordial_data *d, DevState *s, const dengo_options &o, doubl
     p.counter = s->d_counter;
     if (dims <= 0) return 0;
     HIP_OK(hipEventRecord(s->ev0, stream));
+    const int mode = o.precision == DENGO_FP64 ? kFaithful
+                   : o.precision == DENGO_FP64_FAST ? kFast64 : kMixed;
+    void (*kp)(KParams<PrimNet>) = nullptr;
+    void (*kc)(KParams<PrimNet>) = nullptr;
+    if (mode == kFaithful) { kp = k_prim<true, kFaithful>; kc = k_prim<false, kFaithful>; }
+    else if (mode == kFast64) { kp = k_prim<true, kFast64>; kc = k_prim<false, kFast64>; }
+    else { kp = k_prim<true, kMixed>; kc = k_prim<false, kMixed>; }
     if (o.schedule == DENGO_PERSISTENT) {
         int grid = o.grid;
         if (grid <= 0) {
             int per_cu = 0;
-            HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_prim_fp64<true>, kBlock, 0));
+            HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kp, kBlock, 0));
             if (per_cu < 1) per_cu = 1;
             grid = per_cu * s->cu_count;
         }
@@ -362,11 +388,11 @@ int launch_evolve(primordial_data *d, DevState *s, const dengo_options &o, doubl
         if ((long long)grid > need) grid = (int)need;
         hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, stream, s->d_counter);
         HIP_OK(hipGetLastError());
-        hipLaunchKernelGGL(k_prim_fp64<true>, dim3(grid), dim3(kBlock), 0, stream, p);
+        hipLaunchKernelGGL(kp, dim3(grid), dim3(kBlock), 0, stream, p);
     } else {
         const long long blocks = (dims + kBlock - 1) / kBlock;
         if (blocks > 0x7fffffffLL) return fail("too many cells: %lld", dims);
-        hipLaunchKernelGGL(k_prim_fp64<false>, dim3((unsigned)blocks), dim3(kBlock), 0, stream, p);
+        hipLaunchKernelGGL(kc, dim3((unsigned)blocks), dim3(kBlock), 0, stream, p);
     }
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord(s->ev1, stream));
@@ -381,6 +407,9 @@ int check_opts(const dengo_options *opt, dengo_options *o)
     if (o->max_substeps <= 0) o->max_substeps = 50000;
     if (o->schedule != DENGO_PER_CELL && o->schedule != DENGO_PERSISTENT)
         return fail("unknown schedule %d", o->schedule);
+    if (o->precision != DENGO_FP64 && o->precision != DENGO_MIXED &&
+        o->precision != DENGO_FP64_FAST)
+        return fail("unknown precision %d", o->precision);
     return 0;
 }
 

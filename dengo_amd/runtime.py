@@ -16,7 +16,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libdengo_hip.so")
 
-FP64, MIXED = 0, 1
+FP64, MIXED, FP64_FAST = 0, 1, 2
 PER_CELL, PERSISTENT = 0, 1
 
 _dp = ctypes.POINTER(ctypes.c_double)

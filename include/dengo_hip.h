@@ -37,7 +37,12 @@ extern "C" {
 enum dengo_precision {
     DENGO_FP64 = 0,       /* reference-faithful FP64: same operation order and
                              rounding as a plain C build of the algorithm */
-    DENGO_MIXED = 1       /* FP32 LU + FP64 residual (iterative refinement) */
+    DENGO_MIXED = 1,      /* FP32 LU of the row-equilibrated species block,
+                             FP64 residual, state and energy row; the Newton
+                             loop re-evaluates the FP64 residual every sweep */
+    DENGO_FP64_FAST = 2   /* FP64 block solve exploiting the network's
+                             structure (energy column decoupled, structural
+                             zeros skipped, reciprocal pivots, FMA) */
 };
 enum dengo_schedule {
     DENGO_PER_CELL = 0,   /* one cell per lane, grid covers all cells */
