@@ -13,8 +13,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libdengo_hip.so")
-SOURCES = ["dengo_primordial.hip"]
-HEADERS = ["dengo_device.h", "be_driver.h"]
+SOURCES = ["dengo_common.hip", "dengo_primordial.hip", "dengo_oxygen.hip"]
+HEADERS = ["dengo_device.h", "be_driver.h", "be_fast.h", "ma_net.h", "dengo_solver.h", "net_oxygen.h"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -82,13 +82,19 @@ def render_data_header(net, solver_name):
     w("/* Table fields in struct order, for code that walks every table. */")
     fields = ["r_%s" % n for n in _reaction_names(net)] + \
         ["c_%s_%s" % (a, t) for a, t in _cooling_tables(net)]
+    w("/* Field names in solver order, as <name>_setup_data reports them. */")
+    w("#define %s_FIELD_NAMES {%s}" % (solver_name.upper(), ", ".join('"%s"' % s for s in net.species_list())))
     w("#define %s_TABLE_FIELDS(X) \\" % solver_name.upper())
     for i, fld in enumerate(fields):
         w("    X(%s)%s" % (fld, " \\" if i + 1 < len(fields) else ""))
     w("")
+    w('#include "dengo_types.h"')
+    w("")
     w("#ifdef __cplusplus")
     w('extern "C" {')
     w("#endif")
+    w("/* Drop-in entry points (rates_and_rate_tables.C.template:17-54, 149-274,")
+    w("   278-304, 309-350, 460-599; the reference's double &dt is double *dt). */")
     w("%s_data *%s_setup_data(int *NumberOfFields, char ***FieldNames);" % (solver_name, solver_name))
     w("void %s_read_rate_tables(%s_data *data);" % (solver_name, solver_name))
     w("void %s_read_cooling_tables(%s_data *data);" % (solver_name, solver_name))
@@ -98,6 +104,34 @@ def render_data_header(net, solver_name):
     w("int calculate_rhs_%s(double *input, double *rhs, int nstrip, int nchem, void *sdata);" % solver_name)
     w("int calculate_jacobian_%s(double *input, double *Joutput, int nstrip, int nchem, void *sdata);" % solver_name)
     w("void %s_calculate_temperature(%s_data *data, double *input, int nstrip, int nchem);" % (solver_name, solver_name))
+    n = solver_name
+    w("")
+    w("/* Backend extensions (no counterpart in the reference).")
+    w("   Allocate an empty struct; set tables from memory (tables[t * nT + i] in")
+    w("   struct order, T grid log-spaced on [T0, T1]); read the table blob from an")
+    w("   explicit path. */")
+    w("%s_data *%s_alloc_data(void);" % (n, n))
+    w("int %s_set_tables(%s_data *data, const double *tables, int ntables, int nT," % (n, n))
+    w("        double T0, double T1);")
+    w("%s_data *%s_setup_data_from(const char *path, int *NumberOfFields, char ***FieldNames);" % (n, n))
+    w("/* Integrate dims cells held in device memory (AoS [dims][NSPECIES], updated")
+    w("   in place), enqueued on stream (a hipStream_t, NULL = default).  dt0 < 0")
+    w("   selects dtf/1e5.  With ttot_min non-NULL the call synchronises and returns")
+    w("   the smallest elapsed time and that cell's final dt. */")
+    w("int dengo_%s_evolve_device(%s_data *data, const dengo_options *opt, double dtf," % (n, n))
+    w("        double dt0, double z, double *d_input, const double *d_rtol, double *d_atol,")
+    w("        long long dims, const dengo_cell_outputs *outputs, double *ttot_min,")
+    w("        double *dt_min, void *stream);")
+    w("/* Host-array form of dengo_evolve_%s with options and per-cell outputs. */" % n)
+    w("double dengo_%s_evolve_host(%s_data *data, const dengo_options *opt, double dtf," % (n, n))
+    w("        double *dt, double z, double *input, double *rtol, double *atol, long long dims,")
+    w("        int *status, int *nsub, int *nrej, int *nsweep, double *dt_cell, double *T_cell);")
+    w("/* Batched RHS f[dims][N], Jacobian J[dims][N][N] (column-major per cell, may")
+    w("   be NULL), temperature T[dims] and interpolated tables rates[dims][NTABLES]")
+    w("   (may be NULL) for number-density states on host memory; returns the number")
+    w("   of cells whose RHS saw a negative value, or a negative error code. */")
+    w("long long dengo_%s_eval(%s_data *data, const double *u, long long dims, double z," % (n, n))
+    w("        double *f, double *J, double *T, double *rates);")
     w("#ifdef __cplusplus")
     w("}")
     w("static inline double dengo_evolve_%s(double dtf, double &dt, double z, double *input," % solver_name)
@@ -125,3 +159,101 @@ def render_solver(net, solver_name, solver_template, ode_solver_source, output_d
         f.write(render_data_header(net, solver_name))
     written.append(path)
     return written
+
+
+def _monomial(term, net, table_syms, de_name="de"):
+    """Split a sympy monomial  coef * table * [species] * de^p  into
+    (sign*coef, table, species or None, p).  Raises for other forms."""
+    import sympy
+    coef, factors = term.as_coeff_mul()
+    table = species = None
+    depow = 0
+    for f in factors:
+        base, exp = f.as_base_exp()
+        name = str(base)
+        if name in table_syms:
+            if table is not None or exp != 1:
+                raise ValueError("term %s: one table to the first power expected" % term)
+            table = table_syms[name]
+        elif name == de_name:
+            depow += int(exp)
+        elif name in net.species_list():
+            if species is not None or exp != 1:
+                raise ValueError("term %s: one species to the first power expected" % term)
+            species = name
+        else:
+            raise ValueError("term %s: unsupported factor %s" % (term, f))
+    if table is None:
+        raise ValueError("term %s has no table" % term)
+    if depow > 2:
+        raise ValueError("term %s: de power above 2" % term)
+    return float(coef), table, species, depow
+
+
+def render_device_network(net, name):
+    """C++ descriptor of a mass-action network with cooling terms of the form
+    coef * table * species * de^p (ion-by-ion networks), consumed by the
+    generic device network MANet (dengo_amd/csrc/ma_net.h)."""
+    import sympy
+    species = net.species_list()
+    idx = {s: i for i, s in enumerate(species)}
+    tables = net.table_names()
+    tidx = {t: i for i, t in enumerate(tables)}
+    n = len(species)
+    assert species[-1] == net.energy_term.name, "energy must sort last"
+    rx = []
+    for r in net.ordered_reactions():
+        slots = []
+        for c, s in r.left_side:
+            slots += [idx[s.name]] * int(c)
+        prods = {}
+        for c, s in r.right_side:
+            prods[idx[s.name]] = prods.get(idx[s.name], 0) + int(c)
+        if len(slots) > 3 or len(prods) > 4:
+            raise ValueError("reaction %s: at most 3 reactants and 4 products" % r.name)
+        rx.append((tidx[r.name], slots, sorted(prods.items())))
+    cool = []
+    for ca in net.ordered_cooling():
+        syms = {"%s_%s[i]" % (ca.name, t): tidx["%s_%s" % (ca.name, t)] for t in ca.tables}
+        eq = sympy.expand(ca.equation)
+        for term in sympy.Add.make_args(eq):
+            if term == 0:
+                continue
+            coef, t, sp, p = _monomial(term, net, syms)
+            cool.append((t, idx[sp] if sp else -1, p, coef))
+    W = [float(s.weight) for s in net.ordered_species()]
+    FE = [float(getattr(s, "free_electrons", 0.0)) for s in net.ordered_species()]
+    H2 = ["true" if s in ("H2I", "H2II") else "false" for s in species]
+
+    def arr(v):
+        return "{" + ", ".join(repr(x) for x in v) + "}"
+
+    L = ["// Generated by dengo_amd.codegen from the %s network: do not edit." % name,
+         "// Species (solver order): %s" % " ".join(species),
+         "// Tables (struct order): %s" % " ".join(tables),
+         "#pragma once",
+         '#include "ma_net.h"',
+         "",
+         "namespace dengo {",
+         "namespace net_%s {" % name,
+         "struct Desc {",
+         "    static constexpr int N = %d, NTAB = %d, NRXN = %d, NCOOL = %d;" % (
+             n, len(tables), len(rx), len(cool)),
+         "    static constexpr int DE = %d, GE = %d;" % (idx["de"], n - 1),
+         "    static constexpr double W[N] = %s;" % arr(W),
+         "    static constexpr double FREE_E[N] = %s;" % arr(FE),
+         "    static constexpr bool H2G[N] = {%s};" % ", ".join(H2),
+         "    static constexpr Rxn4 RXN[NRXN] = {"]
+    for t, slots, prods in rx:
+        sl = slots + [0] * (3 - len(slots))
+        pc = [c for _, c in prods] + [0] * (4 - len(prods))
+        ps = [s for s, _ in prods] + [0] * (4 - len(prods))
+        L.append("        {%d, %d, {%s}, %d, {%s}, {%s}}," % (
+            t, len(slots), ", ".join(map(str, sl)), len(prods),
+            ", ".join(map(str, pc)), ", ".join(map(str, ps))))
+    L.append("    };")
+    L.append("    static constexpr CoolTerm COOL[NCOOL] = {")
+    for t, sp, p, c in cool:
+        L.append("        {%d, %d, %d, %r}," % (t, sp, p, c))
+    L += ["    };", "};", "}  // namespace net_%s" % name, "}  // namespace dengo", ""]
+    return "\n".join(L)

@@ -317,7 +317,10 @@ DENGO_D void begin_be_fast(Cell<Net> &c)
 
 // Iterative-refinement steps of the FP32 solve (each: one FP64 residual of the
 // block and one FP32 triangular solve).
-constexpr int kRefineSteps = 1;
+#ifndef DENGO_REFINE_STEPS
+#define DENGO_REFINE_STEPS 1
+#endif
+constexpr int kRefineSteps = DENGO_REFINE_STEPS;
 
 template <class Net, class LUT>
 DENGO_D int sweep_fast(Cell<Net> &c, const KParams<Net> &p)

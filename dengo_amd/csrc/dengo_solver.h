@@ -1,0 +1,703 @@
+// Kernels and host implementation of the C ABI, generic over a network.
+//
+// Kernels (one cell per lane, per-cell state in registers, tables in HBM/L2):
+//   k_chem<Net, false, MODE>  one cell per lane, grid covers all cells
+//   k_chem<Net, true, MODE>   persistent: resident lanes pull cells from a
+//                             device counter (one atomic per wave), one Newton
+//                             sweep per loop iteration for every active lane
+//   MODE: kFaithful (reference-faithful FP64), kFast64 (block-structured FP64
+//         solve), kMixed (FP32 LU + FP64 refinement of the species block)
+//   k_eval<Net>               batched RHS / Jacobian / temperature
+// A network instantiation file defines the host traits (struct type, table
+// fields, field names, blob name) and the extern "C" wrappers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stddef.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <type_traits>
+#include <vector>
+
+#include "be_driver.h"
+#include "be_fast.h"
+#include "ma_net.h"
+#include "dengo_types.h"
+
+#ifndef MAX_NCELLS
+#define MAX_NCELLS 1024   // per-strip scratch length of the emitted data struct
+#endif
+
+namespace dengo {
+
+constexpr int kBlock = 64;
+enum { kFaithful = 0, kFast64 = 1, kMixed = 2 };
+
+template <class Net, int MODE>
+DENGO_D bool cell_start(Cell<Net> &c, long long id, const KParams<Net> &p)
+{
+    if constexpr (MODE == kFaithful) return start_or_finish(c, id, p);
+    else return start_or_finish_fast(c, id, p);
+}
+
+template <class Net, int MODE>
+DENGO_D bool cell_sweep(Cell<Net> &c, const KParams<Net> &p)
+{
+    if constexpr (MODE == kFaithful) {
+        const int code = sweep_fp64(c, p);
+        return code != SW_CONTINUE && end_be(c, code, p);
+    } else {
+        using LUT = typename std::conditional<MODE == kMixed, float, double>::type;
+        const int code = sweep_fast<Net, LUT>(c, p);
+        return code != SW_CONTINUE && end_be_fast(c, code, p);
+    }
+}
+
+template <class Net, bool PERSIST, int MODE>
+__global__ void __launch_bounds__(kBlock) k_chem(KParams<Net> p)
+{
+    Cell<Net> c;
+    bool active = false;
+    bool exhausted = false;
+    if (!PERSIST) {
+        const long long id = (long long)blockIdx.x * kBlock + threadIdx.x;
+        if (id < p.ncells) active = cell_start<Net, MODE>(c, id, p);
+        exhausted = true;
+    }
+    const int lane = threadIdx.x & 63;
+    while (true) {
+        if (PERSIST) {
+            // lanes without a cell take the next ones, one atomic per wave
+            const bool need = !active && !exhausted;
+            const unsigned long long mask = __ballot(need);
+            if (mask) {
+                const int leader = __ffsll((long long)mask) - 1;
+                unsigned long long base = 0;
+                if (lane == leader) base = atomicAdd(p.counter, (unsigned long long)__popcll(mask));
+                base = __shfl(base, leader);
+                if (need) {
+                    const unsigned long long rank =
+                        __popcll(mask & ((lane ? (~0ull >> (64 - lane)) : 0ull)));
+                    const unsigned long long id = base + rank;
+                    if (id < (unsigned long long)p.ncells)
+                        active = cell_start<Net, MODE>(c, (long long)id, p);
+                    else
+                        exhausted = true;
+                }
+            }
+            if (!__any(active || !exhausted)) break;
+        } else {
+            if (!__any(active)) break;
+        }
+        if (active && cell_sweep<Net, MODE>(c, p)) {
+            finish_cell(c, p);
+            active = false;
+        }
+    }
+}
+
+template <class Net>
+__global__ void __launch_bounds__(64) k_eval(const double *u_in, long long n,
+                                             typename Net::Env env, double *f_out,
+                                             double *J_out, double *T_out,
+                                             double *rates_out, int *neg_out)
+{
+    constexpr int N = Net::N, NT = Net::NTAB;
+    const long long id = (long long)blockIdx.x * 64 + threadIdx.x;
+    if (id >= n) return;
+    double u[N], f[N], rs[NT], J[N * N];
+    double T;
+#pragma unroll
+    for (int k = 0; k < N; ++k) u[k] = u_in[id * N + k];
+    const bool ok = Net::rhs(u, f, rs, T, env);
+    if (T_out) T_out[id] = T;
+    if (rates_out)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) rates_out[id * NT + t] = rs[t];
+    neg_out[id] = ok ? 0 : 1;
+    if (!ok) return;
+#pragma unroll
+    for (int k = 0; k < N; ++k) f_out[id * N + k] = f[k];
+    if (J_out) {
+        Net::jacobian(u, rs, T, env, J);
+#pragma unroll
+        for (int k = 0; k < N * N; ++k) J_out[id * N * N + k] = J[k];
+    }
+}
+
+template <int DUMMY>
+__global__ void k_reset_counter(unsigned long long *counter)
+{
+    *counter = 0ull;
+}
+
+// Smallest elapsed time over cells (ties: lowest index) and its final dt.
+template <int DUMMY>
+__global__ void __launch_bounds__(1024) k_min_ttot(const double *ttot, const double *dt,
+                                                   long long n, double *out)
+{
+    __shared__ double sv[1024];
+    __shared__ long long si[1024];
+    double best = 1e300;
+    long long bi = -1;
+    for (long long i = threadIdx.x; i < n; i += blockDim.x) {
+        const double v = ttot[i];
+        if (v < best || (v == best && (bi < 0 || i < bi))) {
+            best = v;
+            bi = i;
+        }
+    }
+    sv[threadIdx.x] = best;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            const double v = sv[threadIdx.x + s];
+            const long long j = si[threadIdx.x + s];
+            if (j >= 0 && (si[threadIdx.x] < 0 || v < sv[threadIdx.x] ||
+                           (v == sv[threadIdx.x] && j < si[threadIdx.x]))) {
+                sv[threadIdx.x] = v;
+                si[threadIdx.x] = j;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sv[0];
+        out[1] = si[0] >= 0 ? dt[si[0]] : 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+int set_error(const char *fmt, ...);   // defined in dengo_common.hip
+void set_last_ms(double ms);
+const char *last_error_text();
+
+#define DENGO_HIP_OK(expr)                                                               \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return ::dengo::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                                      __FILE__, __LINE__);                               \
+    } while (0)
+
+struct DevState {
+    int device = -1;
+    double2 *pairs = nullptr;
+    bool stale = true;
+    long long cap = 0;
+    double *d_in = nullptr, *d_rtol = nullptr, *d_atol = nullptr;
+    double *d_ttot = nullptr, *d_dt = nullptr, *d_T = nullptr;
+    int *d_status = nullptr, *d_nsub = nullptr, *d_nrej = nullptr, *d_nsweep = nullptr;
+    unsigned long long *d_counter = nullptr;
+    double *d_red = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int cu_count = 0;
+};
+
+inline int check_options(const dengo_options *opt, dengo_options *o)
+{
+    dengo_default_options(o);
+    if (opt) *o = *opt;
+    if (o->block != 0 && o->block != kBlock) return set_error("block must be 0 or %d", kBlock);
+    if (o->max_substeps <= 0) o->max_substeps = 50000;
+    if (o->schedule != DENGO_PER_CELL && o->schedule != DENGO_PERSISTENT)
+        return set_error("unknown schedule %d", o->schedule);
+    if (o->precision != DENGO_FP64 && o->precision != DENGO_MIXED &&
+        o->precision != DENGO_FP64_FAST)
+        return set_error("unknown precision %d", o->precision);
+    return 0;
+}
+
+// Traits (per network): Data (struct), NSPEC, NTABLES, NT, table_offsets(),
+// field_names(), blob_name().
+template <class Net, class Tr>
+struct Host {
+    using Data = typename Tr::Data;
+    static constexpr int N = Net::N;
+    static constexpr int NTAB = Net::NTAB;
+    static_assert(Tr::NSPEC == N && Tr::NTABLES == NTAB, "traits do not match the network");
+
+    static DevState *state(Data *d)
+    {
+        if (!d->device_state) d->device_state = new DevState();
+        return (DevState *)d->device_state;
+    }
+
+    static const double *table(const Data *d, int t)
+    {
+        return (const double *)((const char *)d + Tr::table_offsets()[t]);
+    }
+
+    static void set_grid(Data *d, double T0, double T1)
+    {
+        // rates_and_rate_tables.C.template:25-36
+        d->bounds[0] = T0;
+        d->bounds[1] = T1;
+        d->nbins = Tr::NT - 1;
+        d->dbin = (log(d->bounds[1]) - log(d->bounds[0])) / d->nbins;
+        d->idbin = (double)(1.0L / d->dbin);
+        d->z_bounds[0] = 0.0;
+        d->z_bounds[1] = 0.0;
+        d->n_zbins = 0;
+        d->d_zbin = 0.0;
+        d->id_zbin = 0.0;
+    }
+
+    static int ensure_device(Data *d, DevState *s)
+    {
+        int dev = 0;
+        DENGO_HIP_OK(hipGetDevice(&dev));
+        if (s->device != dev) {
+            if (s->device >= 0)
+                return set_error("data struct was set up on device %d, current is %d", s->device,
+                                 dev);
+            s->device = dev;
+            DENGO_HIP_OK(hipMalloc(&s->d_counter, sizeof(unsigned long long)));
+            DENGO_HIP_OK(hipMalloc(&s->d_red, 2 * sizeof(double)));
+            DENGO_HIP_OK(hipEventCreate(&s->ev0));
+            DENGO_HIP_OK(hipEventCreate(&s->ev1));
+            hipDeviceProp_t prop;
+            DENGO_HIP_OK(hipGetDeviceProperties(&prop, dev));
+            s->cu_count = prop.multiProcessorCount;
+        }
+        if (s->stale || !s->pairs) {
+            if (d->nbins != Tr::NT - 1) return set_error("tables not set (nbins=%d)", d->nbins);
+            const int nb = d->nbins;
+            std::vector<double2> host((size_t)nb * NTAB);
+            for (int t = 0; t < NTAB; ++t) {
+                const double *r = table(d, t);
+                for (int b = 0; b < nb; ++b) host[(size_t)b * NTAB + t] = make_double2(r[b], r[b + 1]);
+            }
+            if (!s->pairs) DENGO_HIP_OK(hipMalloc(&s->pairs, host.size() * sizeof(double2)));
+            DENGO_HIP_OK(hipMemcpy(s->pairs, host.data(), host.size() * sizeof(double2),
+                                   hipMemcpyHostToDevice));
+            s->stale = false;
+        }
+        return 0;
+    }
+
+    template <class T>
+    static int grow(T **p, long long n)
+    {
+        if (*p) DENGO_HIP_OK(hipFree(*p));
+        *p = nullptr;
+        DENGO_HIP_OK(hipMalloc(p, (size_t)n * sizeof(T)));
+        return 0;
+    }
+
+    static int ensure_scratch(DevState *s, long long n)
+    {
+        if (n <= s->cap) return 0;
+        if (grow(&s->d_in, n * N) || grow(&s->d_rtol, n * N) || grow(&s->d_atol, n * N) ||
+            grow(&s->d_ttot, n) || grow(&s->d_dt, n) || grow(&s->d_T, n) ||
+            grow(&s->d_status, n) || grow(&s->d_nsub, n) || grow(&s->d_nrej, n) ||
+            grow(&s->d_nsweep, n))
+            return -1;
+        s->cap = n;
+        return 0;
+    }
+
+    static typename Net::Env env(const Data *d, const DevState *s, double z)
+    {
+        typename Net::Env e;
+        e.tb.pairs = s->pairs;
+        e.tb.nbins = d->nbins;
+        e.tb.dbin = d->dbin;
+        e.tb.idbin = d->idbin;
+        e.tb.lb = log(d->bounds[0]);
+        e.tb.T0 = d->bounds[0];
+        e.tb.T1 = d->bounds[1];
+        e.zp1 = z + 1.0;
+        e.comp4 = pow(e.zp1, 4);
+        return e;
+    }
+
+    static int read_blob(const char *path, Data *d)
+    {
+        FILE *f = fopen(path, "rb");
+        if (!f) return set_error("cannot open table file %s", path);
+        char magic[8];
+        int32_t nt = 0, nT = 0;
+        double hdr[4];
+        char pad[64 - 8 - 8 - 32];
+        int ok = fread(magic, 1, 8, f) == 8 && fread(&nt, 4, 1, f) == 1 &&
+                 fread(&nT, 4, 1, f) == 1 && fread(hdr, 8, 4, f) == 4 &&
+                 fread(pad, 1, sizeof(pad), f) == sizeof(pad);
+        if (!ok || memcmp(magic, "DENGOTB1", 8) != 0) {
+            fclose(f);
+            return set_error("%s is not a DENGOTB1 table blob", path);
+        }
+        if (nt != NTAB || nT != Tr::NT) {
+            fclose(f);
+            return set_error("%s holds %d tables x %d bins, the struct needs %d x %d", path, nt,
+                             nT, NTAB, Tr::NT);
+        }
+        for (int t = 0; t < nt; ++t) {
+            if (fread((double *)table(d, t), 8, nT, f) != (size_t)nT) {
+                fclose(f);
+                return set_error("%s: short read", path);
+            }
+        }
+        fclose(f);
+        set_grid(d, hdr[0], hdr[1]);
+        state(d)->stale = true;
+        return 0;
+    }
+
+    static void field_names(int *NumberOfFields, char ***FieldNames)
+    {
+        if (FieldNames != NULL && NumberOfFields != NULL) {
+            NumberOfFields[0] = N;
+            FieldNames[0] = new char *[N];
+            for (int i = 0; i < N; ++i) FieldNames[0][i] = strdup(Tr::field_names()[i]);
+        }
+    }
+
+    static Data *alloc() { return (Data *)calloc(1, sizeof(Data)); }
+
+    static int set_tables(Data *d, const double *tables, int ntables, int nT, double T0, double T1)
+    {
+        if (!d || !tables) return set_error("null argument");
+        if (ntables != NTAB || nT != Tr::NT)
+            return set_error("expected %d tables x %d bins, got %d x %d", NTAB, Tr::NT, ntables, nT);
+        if (!(T0 > 0) || !(T1 > T0)) return set_error("bad temperature bounds");
+        for (int t = 0; t < ntables; ++t)
+            memcpy((double *)table(d, t), tables + (size_t)t * nT, sizeof(double) * nT);
+        set_grid(d, T0, T1);
+        state(d)->stale = true;
+        return 0;
+    }
+
+    static Data *setup_from(const char *path, int *NumberOfFields, char ***FieldNames)
+    {
+        Data *d = alloc();
+        if (!d) return NULL;
+        if (read_blob(path, d) != 0) {
+            fprintf(stderr, "%s\n", last_error_text());
+            free(d);
+            return NULL;
+        }
+        field_names(NumberOfFields, FieldNames);
+        return d;
+    }
+
+    static void free_data(Data *d)
+    {
+        if (!d) return;
+        DevState *s = (DevState *)d->device_state;
+        if (s) {
+            void *ptrs[] = {s->pairs, s->d_in, s->d_rtol, s->d_atol, s->d_ttot, s->d_dt, s->d_T,
+                            s->d_status, s->d_nsub, s->d_nrej, s->d_nsweep, s->d_counter,
+                            s->d_red};
+            for (void *q : ptrs)
+                if (q) (void)hipFree(q);
+            if (s->ev0) (void)hipEventDestroy(s->ev0);
+            if (s->ev1) (void)hipEventDestroy(s->ev1);
+            delete s;
+        }
+        free(d);
+    }
+
+    template <int MODE>
+    static void kernels(void (**kp)(KParams<Net>), void (**kc)(KParams<Net>))
+    {
+        *kp = k_chem<Net, true, MODE>;
+        *kc = k_chem<Net, false, MODE>;
+    }
+
+    static int launch(Data *d, DevState *s, const dengo_options &o, double dtf, double dt0,
+                      double z, double *d_input, const double *d_rtol, double *d_atol,
+                      long long dims, const dengo_cell_outputs &out, hipStream_t stream)
+    {
+        KParams<Net> p;
+        p.input = d_input;
+        p.rtol = d_rtol;
+        p.atol = d_atol;
+        p.ncells = dims;
+        p.env = env(d, s, z);
+        p.dtf = dtf;
+        p.dt_in = dt0;
+        p.input_is_number = o.input_is_number;
+        p.max_substeps = o.max_substeps;
+        p.status = out.status;
+        p.nsub = out.nsub;
+        p.nrej = out.nrej;
+        p.nsweep = out.nsweep;
+        p.dt_out = out.dt;
+        p.ttot_out = out.ttot;
+        p.T_out = out.T;
+        p.counter = s->d_counter;
+        if (dims <= 0) return 0;
+        void (*kp)(KParams<Net>) = nullptr;
+        void (*kc)(KParams<Net>) = nullptr;
+        if (o.precision == DENGO_FP64) kernels<kFaithful>(&kp, &kc);
+        else if (o.precision == DENGO_FP64_FAST) kernels<kFast64>(&kp, &kc);
+        else kernels<kMixed>(&kp, &kc);
+        DENGO_HIP_OK(hipEventRecord(s->ev0, stream));
+        if (o.schedule == DENGO_PERSISTENT) {
+            int grid = o.grid;
+            if (grid <= 0) {
+                int per_cu = 0;
+                DENGO_HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kp, kBlock, 0));
+                if (per_cu < 1) per_cu = 1;
+                grid = per_cu * s->cu_count;
+            }
+            const long long need = (dims + kBlock - 1) / kBlock;
+            if ((long long)grid > need) grid = (int)need;
+            hipLaunchKernelGGL(k_reset_counter<0>, dim3(1), dim3(1), 0, stream, s->d_counter);
+            DENGO_HIP_OK(hipGetLastError());
+            hipLaunchKernelGGL(kp, dim3(grid), dim3(kBlock), 0, stream, p);
+        } else {
+            const long long blocks = (dims + kBlock - 1) / kBlock;
+            if (blocks > 0x7fffffffLL) return set_error("too many cells: %lld", dims);
+            hipLaunchKernelGGL(kc, dim3((unsigned)blocks), dim3(kBlock), 0, stream, p);
+        }
+        DENGO_HIP_OK(hipGetLastError());
+        DENGO_HIP_OK(hipEventRecord(s->ev1, stream));
+        return 0;
+    }
+
+    static int evolve_device(Data *d, const dengo_options *opt, double dtf, double dt0, double z,
+                             double *d_input, const double *d_rtol, double *d_atol,
+                             long long dims, const dengo_cell_outputs *outputs,
+                             double *ttot_min, double *dt_min, void *stream)
+    {
+        if (!d || !d_input || !d_rtol || !d_atol) return set_error("null argument");
+        if (dims < 0) return set_error("negative cell count");
+        dengo_options o;
+        if (check_options(opt, &o)) return -1;
+        DevState *s = state(d);
+        if (ensure_device(d, s)) return -1;
+        dengo_cell_outputs out;
+        memset(&out, 0, sizeof(out));
+        if (outputs) out = *outputs;
+        hipStream_t st = (hipStream_t)stream;
+        if (ttot_min && (!out.ttot || !out.dt)) {
+            if (ensure_scratch(s, dims)) return -1;
+            if (!out.ttot) out.ttot = s->d_ttot;
+            if (!out.dt) out.dt = s->d_dt;
+        }
+        d->current_z = z;
+        if (launch(d, s, o, dtf, dt0, z, d_input, d_rtol, d_atol, dims, out, st)) return -1;
+        if (ttot_min) {
+            if (dims > 0) {
+                hipLaunchKernelGGL(k_min_ttot<0>, dim3(1), dim3(1024), 0, st, out.ttot, out.dt,
+                                   dims, s->d_red);
+                DENGO_HIP_OK(hipGetLastError());
+                double red[2];
+                DENGO_HIP_OK(hipMemcpyAsync(red, s->d_red, sizeof(red), hipMemcpyDeviceToHost, st));
+                DENGO_HIP_OK(hipStreamSynchronize(st));
+                *ttot_min = red[0];
+                if (dt_min) *dt_min = red[1];
+                float ms = 0.f;
+                if (hipEventElapsedTime(&ms, s->ev0, s->ev1) == hipSuccess) set_last_ms(ms);
+            } else {
+                *ttot_min = 0.0;
+                if (dt_min) *dt_min = dt0;
+            }
+        }
+        return 0;
+    }
+
+    static double evolve_host(Data *d, const dengo_options *opt, double dtf, double *dt, double z,
+                              double *input, double *rtol, double *atol, long long dims,
+                              int *status, int *nsub, int *nrej, int *nsweep, double *dt_cell,
+                              double *T_cell)
+    {
+        if (!d || !dt || !input || !rtol || !atol) {
+            set_error("null argument");
+            return -1.0;
+        }
+        if (dims <= 0) return 0.0;
+        DevState *s = state(d);
+        if (ensure_device(d, s) || ensure_scratch(s, dims)) return -1.0;
+        const size_t bytes = (size_t)dims * N * sizeof(double);
+        if (hipMemcpy(s->d_in, input, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(s->d_rtol, rtol, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(s->d_atol, atol, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+            set_error("host-to-device copy failed");
+            return -1.0;
+        }
+        dengo_cell_outputs out;
+        out.status = s->d_status;
+        out.nsub = s->d_nsub;
+        out.nrej = s->d_nrej;
+        out.nsweep = s->d_nsweep;
+        out.dt = s->d_dt;
+        out.ttot = s->d_ttot;
+        out.T = s->d_T;
+        double tmin = 0.0, dtm = 0.0;
+        if (evolve_device(d, opt, dtf, *dt, z, s->d_in, s->d_rtol, s->d_atol, dims, &out, &tmin,
+                          &dtm, NULL) != 0)
+            return -1.0;
+        struct Copy { void *dst; const void *src; size_t n; };
+        const long long nts = dims < MAX_NCELLS ? dims : MAX_NCELLS;
+        const Copy cps[] = {{input, s->d_in, bytes}, {atol, s->d_atol, bytes},
+                            {status, s->d_status, dims * sizeof(int)},
+                            {nsub, s->d_nsub, dims * sizeof(int)},
+                            {nrej, s->d_nrej, dims * sizeof(int)},
+                            {nsweep, s->d_nsweep, dims * sizeof(int)},
+                            {dt_cell, s->d_dt, dims * sizeof(double)},
+                            {T_cell, s->d_T, dims * sizeof(double)},
+                            // the reference leaves the last strip's temperatures in data->Ts
+                            {d->Ts, s->d_T, nts * sizeof(double)}};
+        for (const Copy &c : cps) {
+            if (c.dst && hipMemcpy(c.dst, c.src, c.n, hipMemcpyDeviceToHost) != hipSuccess) {
+                set_error("device-to-host copy failed");
+                return -1.0;
+            }
+        }
+        *dt = dtm;
+        return tmin;
+    }
+
+    static long long eval(Data *d, const double *u, long long dims, double z, double *f,
+                          double *J, double *T, double *rates)
+    {
+        if (!d || !u || !f) return set_error("null argument");
+        if (dims <= 0) return 0;
+        DevState *s = state(d);
+        if (ensure_device(d, s)) return -1;
+        const size_t bu = (size_t)dims * N * sizeof(double);
+        double *du = nullptr, *df = nullptr, *dJ = nullptr, *dT = nullptr, *dr = nullptr;
+        int *dn = nullptr;
+        DENGO_HIP_OK(hipMalloc(&du, bu));
+        DENGO_HIP_OK(hipMalloc(&df, bu));
+        DENGO_HIP_OK(hipMalloc(&dn, dims * sizeof(int)));
+        if (J) DENGO_HIP_OK(hipMalloc(&dJ, bu * N));
+        if (T) DENGO_HIP_OK(hipMalloc(&dT, dims * sizeof(double)));
+        if (rates) DENGO_HIP_OK(hipMalloc(&dr, (size_t)dims * NTAB * sizeof(double)));
+        DENGO_HIP_OK(hipMemcpy(du, u, bu, hipMemcpyHostToDevice));
+        DENGO_HIP_OK(hipMemset(df, 0, bu));
+        const long long blocks = (dims + 63) / 64;
+        hipLaunchKernelGGL(k_eval<Net>, dim3((unsigned)blocks), dim3(64), 0, 0, du, dims,
+                           env(d, s, z), df, dJ, dT, dr, dn);
+        DENGO_HIP_OK(hipGetLastError());
+        std::vector<int> neg(dims);
+        DENGO_HIP_OK(hipMemcpy(neg.data(), dn, dims * sizeof(int), hipMemcpyDeviceToHost));
+        DENGO_HIP_OK(hipMemcpy(f, df, bu, hipMemcpyDeviceToHost));
+        if (J) DENGO_HIP_OK(hipMemcpy(J, dJ, bu * N, hipMemcpyDeviceToHost));
+        if (T) DENGO_HIP_OK(hipMemcpy(T, dT, dims * sizeof(double), hipMemcpyDeviceToHost));
+        if (rates)
+            DENGO_HIP_OK(hipMemcpy(rates, dr, (size_t)dims * NTAB * sizeof(double),
+                                   hipMemcpyDeviceToHost));
+        void *ptrs[] = {du, df, dJ, dT, dr, dn};
+        for (void *q : ptrs)
+            if (q) DENGO_HIP_OK(hipFree(q));
+        long long nneg = 0;
+        for (long long i = 0; i < dims; ++i) nneg += neg[i];
+        return nneg;
+    }
+
+    static int calculate_rhs(double *input, double *rhs, int nstrip, int nchem, void *sdata)
+    {
+        Data *d = (Data *)sdata;
+        if (nchem != N) return 1;
+        std::vector<double> T(nstrip);
+        const long long r = eval(d, input, nstrip, d->current_z, rhs, NULL, T.data(), NULL);
+        if (r < 0) {
+            fprintf(stderr, "calculate_rhs: %s\n", last_error_text());
+            return 1;
+        }
+        for (int i = 0; i < nstrip && i < MAX_NCELLS; ++i) d->Ts[i] = T[i];
+        return r > 0 ? 1 : 0;
+    }
+
+    static int calculate_jacobian(double *input, double *Joutput, int nstrip, int nchem,
+                                  void *sdata)
+    {
+        Data *d = (Data *)sdata;
+        if (nchem != N) return 1;
+        std::vector<double> f((size_t)nstrip * nchem);
+        const long long r = eval(d, input, nstrip, d->current_z, f.data(), Joutput, NULL, NULL);
+        if (r < 0) {
+            fprintf(stderr, "calculate_jacobian: %s\n", last_error_text());
+            return 1;
+        }
+        return r > 0 ? 1 : 0;
+    }
+
+    static void calculate_temperature(Data *d, double *input, int nstrip, int nchem)
+    {
+        if (nchem != N || nstrip <= 0) return;
+        std::vector<double> f((size_t)nstrip * nchem), T(nstrip);
+        if (eval(d, input, nstrip, d->current_z, f.data(), NULL, T.data(), NULL) < 0) {
+            fprintf(stderr, "calculate_temperature: %s\n", last_error_text());
+            return;
+        }
+        for (int i = 0; i < nstrip && i < MAX_NCELLS; ++i) d->Ts[i] = T[i];
+    }
+};
+
+}  // namespace dengo
+
+// extern "C" wrappers of one network: <n>_setup_data, dengo_evolve_<n>, ...
+#define DENGO_DEFINE_C_ABI(n, H)                                                               \
+    extern "C" {                                                                               \
+    n##_data *n##_alloc_data(void) { return H::alloc(); }                                      \
+    int n##_set_tables(n##_data *d, const double *t, int nt, int nT, double T0, double T1)     \
+    {                                                                                          \
+        return H::set_tables(d, t, nt, nT, T0, T1);                                            \
+    }                                                                                          \
+    n##_data *n##_setup_data_from(const char *path, int *nf, char ***fn)                       \
+    {                                                                                          \
+        return H::setup_from(path, nf, fn);                                                    \
+    }                                                                                          \
+    n##_data *n##_setup_data(int *nf, char ***fn) { return H::setup_from(#n "_tables.bin", nf, fn); } \
+    void n##_read_rate_tables(n##_data *d)                                                     \
+    {                                                                                          \
+        if (H::read_blob(#n "_tables.bin", d) != 0) fprintf(stderr, "%s\n", dengo_last_error()); \
+    }                                                                                          \
+    void n##_read_cooling_tables(n##_data *d)                                                  \
+    {                                                                                          \
+        if (H::read_blob(#n "_tables.bin", d) != 0) fprintf(stderr, "%s\n", dengo_last_error()); \
+    }                                                                                          \
+    void n##_free_data(n##_data *d) { H::free_data(d); }                                       \
+    int dengo_##n##_evolve_device(n##_data *d, const dengo_options *o, double dtf, double dt0,  \
+                                  double z, double *x, const double *rt, double *at,           \
+                                  long long dims, const dengo_cell_outputs *out,              \
+                                  double *tmin, double *dtmin, void *stream)                  \
+    {                                                                                          \
+        return H::evolve_device(d, o, dtf, dt0, z, x, rt, at, dims, out, tmin, dtmin, stream); \
+    }                                                                                          \
+    double dengo_##n##_evolve_host(n##_data *d, const dengo_options *o, double dtf, double *dt, \
+                                   double z, double *x, double *rt, double *at, long long dims, \
+                                   int *st, int *ns, int *nr, int *nw, double *dtc, double *Tc) \
+    {                                                                                          \
+        return H::evolve_host(d, o, dtf, dt, z, x, rt, at, dims, st, ns, nr, nw, dtc, Tc);     \
+    }                                                                                          \
+    double dengo_evolve_##n(double dtf, double *dt, double z, double *input, double *rtol,     \
+                            double *atol, long long dims, n##_data *data)                      \
+    {                                                                                          \
+        dengo_options o;                                                                       \
+        dengo_default_options(&o);                                                             \
+        const double t = H::evolve_host(data, &o, dtf, dt, z, input, rtol, atol, dims, NULL,   \
+                                        NULL, NULL, NULL, NULL, NULL);                         \
+        if (t < 0) fprintf(stderr, "dengo_evolve_" #n ": %s\n", dengo_last_error());           \
+        return t;                                                                              \
+    }                                                                                          \
+    long long dengo_##n##_eval(n##_data *d, const double *u, long long dims, double z,         \
+                               double *f, double *J, double *T, double *rates)                 \
+    {                                                                                          \
+        return H::eval(d, u, dims, z, f, J, T, rates);                                         \
+    }                                                                                          \
+    int calculate_rhs_##n(double *in, double *rhs, int ns, int nc, void *sd)                   \
+    {                                                                                          \
+        return H::calculate_rhs(in, rhs, ns, nc, sd);                                          \
+    }                                                                                          \
+    int calculate_jacobian_##n(double *in, double *J, int ns, int nc, void *sd)                \
+    {                                                                                          \
+        return H::calculate_jacobian(in, J, ns, nc, sd);                                       \
+    }                                                                                          \
+    void n##_calculate_temperature(n##_data *d, double *in, int ns, int nc)                    \
+    {                                                                                          \
+        H::calculate_temperature(d, in, ns, nc);                                               \
+    }                                                                                          \
+    }

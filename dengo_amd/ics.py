@@ -76,3 +76,41 @@ def tolerances(inp, rtol=1e-9, atol_rel=1e-9):
     """rtol = 1e-9 and atol = 1e-9 * initial value, as the emitted main sets
     them (rates_and_rate_tables.C.template:97-98)."""
     return (np.full_like(inp, rtol), np.ascontiguousarray(inp * atol_rel))
+
+
+OXYGEN_WEIGHT = 15.9994
+
+
+def ion_temperature_ge(n, T, weight, ge_index):
+    """ge that makes the solver's temperature equal T for an ion chain (all
+    gamma = 5/3; density excludes de and ge, the gamma sum includes de)."""
+    chem = n[:, :ge_index - 1]
+    density = np.zeros(n.shape[0])
+    for k in range(chem.shape[1]):
+        density = density + weight * chem[:, k]
+    gsum = np.zeros(n.shape[0])
+    for k in range(ge_index):
+        gsum = gsum + n[:, k] / (GAMMA - 1.0)
+    return T * KB_SOLVER * gsum / (density * MH_SOLVER)
+
+
+def oxygen_ics(ncells, seed, nO_range=(1e-4, 1e2), T_range=(1e4, 1e8)):
+    """O I-IX cells: oxygen number density log-uniform, T log-uniform 1e4-1e8 K,
+    ion fractions Dirichlet(1, ..., 1) floored at 1e-20, electrons from charge
+    neutrality, ge from the solver's temperature formula.  Returns
+    (input in mass densities, number densities, T)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nO = _logu(rng, nO_range[0], nO_range[1], ncells)
+    T = _logu(rng, T_range[0], T_range[1], ncells)
+    frac = np.maximum(rng.dirichlet(np.ones(9), ncells), 1e-20)
+    n = np.zeros((ncells, 11))
+    n[:, :9] = frac * nO[:, None]
+    total_e = np.zeros(ncells)
+    for k in range(9):
+        total_e = total_e + n[:, k] * float(k)
+    n[:, 9] = total_e
+    n[:, 10] = ion_temperature_ge(n, T, OXYGEN_WEIGHT, 10)
+    inp = n.copy()
+    inp[:, :10] = n[:, :10] * np.append(np.full(9, OXYGEN_WEIGHT), 1.0)[None, :] * 1.67e-24
+    assert np.all(inp > 0) and np.all(np.isfinite(inp))
+    return np.ascontiguousarray(inp), n, T

@@ -43,3 +43,35 @@ def primordial_tables(T_bounds=(1.0, 1e9), n_bins=1024):
             _TABLE_CACHE[key] = (net.table_names(), net.table_block())
     names, block = _TABLE_CACHE[key]
     return list(names), block.copy()
+
+
+OXYGEN_SPECIES = tuple("O_%d" % k for k in range(1, 10)) + ("de", "ge")
+
+
+def oxygen_network(seed=3, T_bounds=(1e4, 1e9), n_bins=1024, stop_time=1e12):
+    """O I - O IX ion-by-ion network (11 unknowns, 16 reactions, 9 cooling
+    tables) with smooth synthetic tables on 1024 log-T bins over 1e4-1e9 K:
+    rates in 1e-16..1e-8 cm^3/s, cooling in 1e-25..1e-21 erg cm^3/s."""
+    from .ion_by_ion import setup_ionization, synthetic_ion_tables
+    fns = synthetic_ion_tables("O", seed)
+    setup_ionization("O", fns)
+    net = ChemicalNetwork(stop_time=stop_time)
+    for k in range(1, 10):
+        net.add_cooling("O_%d_c" % k)
+    for k in range(1, 10):
+        if k < 9:
+            net.add_reaction("O_%d_i" % k)
+        if k > 1:
+            net.add_reaction("O_%d_r" % k)
+    net.init_temperature(T_bounds, n_bins)
+    assert tuple(net.species_list()) == OXYGEN_SPECIES, net.species_list()
+    return net
+
+
+def oxygen_tables(seed=3, T_bounds=(1e4, 1e9), n_bins=1024):
+    key = ("oxygen", seed, float(T_bounds[0]), float(T_bounds[1]), int(n_bins))
+    if key not in _TABLE_CACHE:
+        net = oxygen_network(seed, T_bounds, n_bins)
+        _TABLE_CACHE[key] = (net.table_names(), net.table_block())
+    names, block = _TABLE_CACHE[key]
+    return list(names), block.copy()

@@ -37,40 +37,51 @@ class CellOutputs(ctypes.Structure):
                 ("dt", ctypes.c_void_p), ("ttot", ctypes.c_void_p), ("T", ctypes.c_void_p)]
 
 
+NETWORKS = ("primordial", "oxygen")
+
+
+def _network_signatures(n):
+    return {
+        "%s_setup_data" % n: (ctypes.c_void_p, [ctypes.c_void_p, ctypes.c_void_p]),
+        "%s_read_rate_tables" % n: (None, [ctypes.c_void_p]),
+        "%s_read_cooling_tables" % n: (None, [ctypes.c_void_p]),
+        "%s_free_data" % n: (None, [ctypes.c_void_p]),
+        "dengo_evolve_%s" % n: (ctypes.c_double, [ctypes.c_double, _dp, ctypes.c_double, _dp,
+                                                  _dp, _dp, ctypes.c_longlong,
+                                                  ctypes.c_void_p]),
+        "calculate_rhs_%s" % n: (ctypes.c_int, [_dp, _dp, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_void_p]),
+        "calculate_jacobian_%s" % n: (ctypes.c_int, [_dp, _dp, ctypes.c_int, ctypes.c_int,
+                                                     ctypes.c_void_p]),
+        "%s_calculate_temperature" % n: (None, [ctypes.c_void_p, _dp, ctypes.c_int,
+                                                ctypes.c_int]),
+        "%s_alloc_data" % n: (ctypes.c_void_p, []),
+        "%s_set_tables" % n: (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_double, ctypes.c_double]),
+        "%s_setup_data_from" % n: (ctypes.c_void_p, [ctypes.c_char_p, ctypes.c_void_p,
+                                                     ctypes.c_void_p]),
+        "dengo_%s_evolve_device" % n: (
+            ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Options), ctypes.c_double,
+                           ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
+                           ctypes.c_void_p, ctypes.c_longlong, ctypes.POINTER(CellOutputs),
+                           _dp, _dp, ctypes.c_void_p]),
+        "dengo_%s_evolve_host" % n: (
+            ctypes.c_double, [ctypes.c_void_p, ctypes.POINTER(Options), ctypes.c_double, _dp,
+                              ctypes.c_double, _dp, _dp, _dp, ctypes.c_longlong, _ip, _ip, _ip,
+                              _ip, _dp, _dp]),
+        "dengo_%s_eval" % n: (ctypes.c_longlong, [ctypes.c_void_p, _dp, ctypes.c_longlong,
+                                                  ctypes.c_double, _dp, _dp, _dp, _dp]),
+    }
+
+
 # Every symbol include/*.h declares, with its ctypes signature.
 SIGNATURES = {
-    "primordial_setup_data": (ctypes.c_void_p, [ctypes.c_void_p, ctypes.c_void_p]),
-    "primordial_read_rate_tables": (None, [ctypes.c_void_p]),
-    "primordial_read_cooling_tables": (None, [ctypes.c_void_p]),
-    "primordial_free_data": (None, [ctypes.c_void_p]),
-    "dengo_evolve_primordial": (ctypes.c_double, [ctypes.c_double, _dp, ctypes.c_double, _dp,
-                                                  _dp, _dp, ctypes.c_longlong, ctypes.c_void_p]),
-    "calculate_rhs_primordial": (ctypes.c_int, [_dp, _dp, ctypes.c_int, ctypes.c_int,
-                                                ctypes.c_void_p]),
-    "calculate_jacobian_primordial": (ctypes.c_int, [_dp, _dp, ctypes.c_int, ctypes.c_int,
-                                                     ctypes.c_void_p]),
-    "primordial_calculate_temperature": (None, [ctypes.c_void_p, _dp, ctypes.c_int,
-                                                ctypes.c_int]),
     "dengo_default_options": (None, [ctypes.POINTER(Options)]),
     "dengo_last_error": (ctypes.c_char_p, []),
-    "primordial_alloc_data": (ctypes.c_void_p, []),
-    "primordial_set_tables": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int, ctypes.c_int,
-                                             ctypes.c_double, ctypes.c_double]),
-    "primordial_setup_data_from": (ctypes.c_void_p, [ctypes.c_char_p, ctypes.c_void_p,
-                                                     ctypes.c_void_p]),
-    "dengo_primordial_evolve_device": (
-        ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Options), ctypes.c_double,
-                       ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
-                       ctypes.c_void_p, ctypes.c_longlong, ctypes.POINTER(CellOutputs),
-                       _dp, _dp, ctypes.c_void_p]),
-    "dengo_primordial_evolve_host": (
-        ctypes.c_double, [ctypes.c_void_p, ctypes.POINTER(Options), ctypes.c_double, _dp,
-                          ctypes.c_double, _dp, _dp, _dp, ctypes.c_longlong, _ip, _ip, _ip,
-                          _ip, _dp, _dp]),
-    "dengo_primordial_eval": (ctypes.c_longlong, [ctypes.c_void_p, _dp, ctypes.c_longlong,
-                                                  ctypes.c_double, _dp, _dp, _dp, _dp]),
     "dengo_last_kernel_ms": (ctypes.c_double, []),
 }
+for _n in NETWORKS:
+    SIGNATURES.update(_network_signatures(_n))
 
 
 def load(path=LIB_PATH):
@@ -136,29 +147,28 @@ def write_table_blob(path, block, T_bounds, z_bounds=(0.0, 0.0)):
         f.write(block.tobytes())
 
 
-class PrimordialSolver(object):
-    """One primordial_data struct with its tables resident on the GPU."""
+class NetworkSolver(object):
+    """One <name>_data struct (a built-in network) with its tables resident on
+    the GPU."""
 
-    N = 10
-
-    def __init__(self, tables=None, T_bounds=(1.0, 1e9)):
+    def __init__(self, name, tables, T_bounds):
         lib = load()
-        if tables is None:
-            from .networks import primordial_tables
-            _, tables = primordial_tables(T_bounds)
+        if name not in NETWORKS:
+            raise ValueError("unknown network %r (built in: %s)" % (name, ", ".join(NETWORKS)))
+        self.name = name
         tables = np.ascontiguousarray(tables, dtype=np.float64)
         self.tables = tables
         self.T_bounds = T_bounds
-        self.data = lib.primordial_alloc_data()
+        self._fn = lambda pattern: getattr(lib, pattern % name)
+        self.data = self._fn("%s_alloc_data")()
         if not self.data:
-            raise MemoryError("primordial_alloc_data")
-        _check(lib.primordial_set_tables(self.data, _p(tables), tables.shape[0],
-                                         tables.shape[1], T_bounds[0], T_bounds[1]),
-               "primordial_set_tables")
+            raise MemoryError("%s_alloc_data" % name)
+        _check(self._fn("%s_set_tables")(self.data, _p(tables), tables.shape[0], tables.shape[1],
+                                         T_bounds[0], T_bounds[1]), "%s_set_tables" % name)
 
     def close(self):
-        if self.data:
-            load().primordial_free_data(self.data)
+        if getattr(self, "data", None):
+            self._fn("%s_free_data")(self.data)
             self.data = None
 
     def __del__(self):
@@ -170,33 +180,33 @@ class PrimordialSolver(object):
     def evolve(self, inp, rtol, atol, dtf, dt=-1.0, z=-1.0, **opt):
         """dengo_evolve semantics on host copies; returns a dict with the final
         state, atol, ttot, dt and per-cell status/nsub/nrej/nsweep/dt/T."""
-        lib = load()
         out = np.array(inp, dtype=np.float64, copy=True, order="C")
         at = np.array(atol, dtype=np.float64, copy=True, order="C")
         rt = np.array(rtol, dtype=np.float64, copy=True, order="C")
+        if out.ndim != 2 or out.shape[1] != self.N:
+            raise ValueError("expected inputs of shape [n, %d]" % self.N)
         n = out.shape[0]
         res = dict(status=np.zeros(n, np.int32), nsub=np.zeros(n, np.int32),
                    nrej=np.zeros(n, np.int32), nsweep=np.zeros(n, np.int32),
                    dt_cell=np.zeros(n), T=np.zeros(n))
         o = options(**opt)
         dtc = ctypes.c_double(dt)
-        ttot = lib.dengo_primordial_evolve_host(
+        ttot = self._fn("dengo_%s_evolve_host")(
             self.data, ctypes.byref(o), dtf, ctypes.byref(dtc), z, _p(out), _p(rt), _p(at), n,
             _p(res["status"], _ip), _p(res["nsub"], _ip), _p(res["nrej"], _ip),
             _p(res["nsweep"], _ip), _p(res["dt_cell"]), _p(res["T"]))
         if ttot < 0:
             raise RuntimeError("evolve failed: %s" % last_error())
         res.update(state=out, atol=at, ttot=ttot, dt=dtc.value,
-                   kernel_ms=lib.dengo_last_kernel_ms())
+                   kernel_ms=load().dengo_last_kernel_ms())
         return res
 
     def evolve_device(self, inp, rtol, atol, dtf, dt=-1.0, z=-1.0, outputs=None,
                       stream=None, sync_min=True, **opt):
         """In-place integration of device-resident float64 torch tensors
-        [ncells, 10].  outputs: optional dict of device tensors (status, nsub,
+        [ncells, N].  outputs: optional dict of device tensors (status, nsub,
         nrej, nsweep: int32; dt, ttot, T: float64).  Returns (ttot_min, dt_min)
         when sync_min, else None (asynchronous on `stream`)."""
-        lib = load()
         for t in (inp, rtol, atol):
             if not (t.is_cuda and t.dtype.is_floating_point and t.element_size() == 8
                     and t.is_contiguous() and t.dim() == 2 and t.shape[1] == self.N):
@@ -204,30 +214,52 @@ class PrimordialSolver(object):
         n = inp.shape[0]
         co = CellOutputs()
         for k, v in (outputs or {}).items():
+            if v.numel() < n:
+                raise ValueError("output %s too small" % k)
             setattr(co, k, v.data_ptr())
         o = options(**opt)
         tmin = ctypes.c_double(0.0)
         dmin = ctypes.c_double(0.0)
         s = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
-        rc = lib.dengo_primordial_evolve_device(
+        rc = self._fn("dengo_%s_evolve_device")(
             self.data, ctypes.byref(o), dtf, dt, z, ctypes.c_void_p(inp.data_ptr()),
             ctypes.c_void_p(rtol.data_ptr()), ctypes.c_void_p(atol.data_ptr()), n,
             ctypes.byref(co), ctypes.byref(tmin) if sync_min else None,
             ctypes.byref(dmin) if sync_min else None, s)
-        _check(rc, "dengo_primordial_evolve_device")
+        _check(rc, "dengo_%s_evolve_device" % self.name)
         return (tmin.value, dmin.value) if sync_min else None
 
     def eval(self, u, z=-1.0, jacobian=True):
-        """RHS f[n,10], Jacobian J[n,10,10] (J[c, col, row]), T[n], rates[n,45]
+        """RHS f[n,N], Jacobian J[n,N,N] (J[c, col, row]), T[n], rates[n,NTAB]
         and the count of cells whose RHS saw a negative value."""
-        lib = load()
         u = np.ascontiguousarray(u, dtype=np.float64)
         n = u.shape[0]
-        f = np.zeros((n, 10))
-        J = np.zeros((n, 10, 10)) if jacobian else None
+        N = self.N
+        f = np.zeros((n, N))
+        J = np.zeros((n, N, N)) if jacobian else None
         T = np.zeros(n)
         rates = np.zeros((n, self.tables.shape[0]))
-        r = lib.dengo_primordial_eval(self.data, _p(u), n, z, _p(f), _p(J), _p(T), _p(rates))
+        r = self._fn("dengo_%s_eval")(self.data, _p(u), n, z, _p(f), _p(J), _p(T), _p(rates))
         if r < 0:
             raise RuntimeError("eval failed: %s" % last_error())
         return f, J, T, rates, int(r)
+
+
+class PrimordialSolver(NetworkSolver):
+    N = 10
+
+    def __init__(self, tables=None, T_bounds=(1.0, 1e9)):
+        if tables is None:
+            from .networks import primordial_tables
+            _, tables = primordial_tables(T_bounds)
+        NetworkSolver.__init__(self, "primordial", tables, T_bounds)
+
+
+class OxygenSolver(NetworkSolver):
+    N = 11
+
+    def __init__(self, tables=None, T_bounds=(1e4, 1e9)):
+        if tables is None:
+            from .networks import oxygen_tables
+            _, tables = oxygen_tables(T_bounds=T_bounds)
+        NetworkSolver.__init__(self, "oxygen", tables, T_bounds)

@@ -1,0 +1,195 @@
+/* Drop-in header for the oxygen network (generated by dengo_amd.codegen).
+   Struct fields and order follow dengo/templates/rates_and_rate_tables.h.template:30-83;
+   entry points follow :85-108 with C linkage. */
+#ifndef OXYGEN_SOLVER_H
+#define OXYGEN_SOLVER_H
+
+#ifndef MAX_NCELLS
+#define MAX_NCELLS 1024
+#endif
+#define OXYGEN_NSPECIES 11
+#define OXYGEN_NTABLES 25
+#define OXYGEN_NT 1024
+
+typedef struct oxygen_data_ {
+    double dbin;
+    double idbin;
+    double bounds[2];
+    int nbins;
+    double d_zbin;
+    double id_zbin;
+    double z_bounds[2];
+    int n_zbins;
+    double current_z;
+    double zdef;
+    double dz;
+    double Ts[MAX_NCELLS];
+    double Tdef[MAX_NCELLS];
+    double dT[MAX_NCELLS];
+    double logTs[MAX_NCELLS];
+    double invTs[MAX_NCELLS];
+    double dTs_ge[MAX_NCELLS];
+    double r_O_1_i[1024];
+    double rs_O_1_i[MAX_NCELLS];
+    double drs_O_1_i[MAX_NCELLS];
+    double r_O_2_i[1024];
+    double rs_O_2_i[MAX_NCELLS];
+    double drs_O_2_i[MAX_NCELLS];
+    double r_O_2_r[1024];
+    double rs_O_2_r[MAX_NCELLS];
+    double drs_O_2_r[MAX_NCELLS];
+    double r_O_3_i[1024];
+    double rs_O_3_i[MAX_NCELLS];
+    double drs_O_3_i[MAX_NCELLS];
+    double r_O_3_r[1024];
+    double rs_O_3_r[MAX_NCELLS];
+    double drs_O_3_r[MAX_NCELLS];
+    double r_O_4_i[1024];
+    double rs_O_4_i[MAX_NCELLS];
+    double drs_O_4_i[MAX_NCELLS];
+    double r_O_4_r[1024];
+    double rs_O_4_r[MAX_NCELLS];
+    double drs_O_4_r[MAX_NCELLS];
+    double r_O_5_i[1024];
+    double rs_O_5_i[MAX_NCELLS];
+    double drs_O_5_i[MAX_NCELLS];
+    double r_O_5_r[1024];
+    double rs_O_5_r[MAX_NCELLS];
+    double drs_O_5_r[MAX_NCELLS];
+    double r_O_6_i[1024];
+    double rs_O_6_i[MAX_NCELLS];
+    double drs_O_6_i[MAX_NCELLS];
+    double r_O_6_r[1024];
+    double rs_O_6_r[MAX_NCELLS];
+    double drs_O_6_r[MAX_NCELLS];
+    double r_O_7_i[1024];
+    double rs_O_7_i[MAX_NCELLS];
+    double drs_O_7_i[MAX_NCELLS];
+    double r_O_7_r[1024];
+    double rs_O_7_r[MAX_NCELLS];
+    double drs_O_7_r[MAX_NCELLS];
+    double r_O_8_i[1024];
+    double rs_O_8_i[MAX_NCELLS];
+    double drs_O_8_i[MAX_NCELLS];
+    double r_O_8_r[1024];
+    double rs_O_8_r[MAX_NCELLS];
+    double drs_O_8_r[MAX_NCELLS];
+    double r_O_9_r[1024];
+    double rs_O_9_r[MAX_NCELLS];
+    double drs_O_9_r[MAX_NCELLS];
+    double c_O_1_c_O_1_c[1024];
+    double cs_O_1_c_O_1_c[MAX_NCELLS];
+    double dcs_O_1_c_O_1_c[MAX_NCELLS];
+    double c_O_2_c_O_2_c[1024];
+    double cs_O_2_c_O_2_c[MAX_NCELLS];
+    double dcs_O_2_c_O_2_c[MAX_NCELLS];
+    double c_O_3_c_O_3_c[1024];
+    double cs_O_3_c_O_3_c[MAX_NCELLS];
+    double dcs_O_3_c_O_3_c[MAX_NCELLS];
+    double c_O_4_c_O_4_c[1024];
+    double cs_O_4_c_O_4_c[MAX_NCELLS];
+    double dcs_O_4_c_O_4_c[MAX_NCELLS];
+    double c_O_5_c_O_5_c[1024];
+    double cs_O_5_c_O_5_c[MAX_NCELLS];
+    double dcs_O_5_c_O_5_c[MAX_NCELLS];
+    double c_O_6_c_O_6_c[1024];
+    double cs_O_6_c_O_6_c[MAX_NCELLS];
+    double dcs_O_6_c_O_6_c[MAX_NCELLS];
+    double c_O_7_c_O_7_c[1024];
+    double cs_O_7_c_O_7_c[MAX_NCELLS];
+    double dcs_O_7_c_O_7_c[MAX_NCELLS];
+    double c_O_8_c_O_8_c[1024];
+    double cs_O_8_c_O_8_c[MAX_NCELLS];
+    double dcs_O_8_c_O_8_c[MAX_NCELLS];
+    double c_O_9_c_O_9_c[1024];
+    double cs_O_9_c_O_9_c[MAX_NCELLS];
+    double dcs_O_9_c_O_9_c[MAX_NCELLS];
+    int bin_id[MAX_NCELLS];
+    int ncells;
+    /* Device-resident tables and scratch, owned by the HIP library. */
+    void *device_state;
+} oxygen_data;
+
+/* Table fields in struct order, for code that walks every table. */
+/* Field names in solver order, as <name>_setup_data reports them. */
+#define OXYGEN_FIELD_NAMES {"O_1", "O_2", "O_3", "O_4", "O_5", "O_6", "O_7", "O_8", "O_9", "de", "ge"}
+#define OXYGEN_TABLE_FIELDS(X) \
+    X(r_O_1_i) \
+    X(r_O_2_i) \
+    X(r_O_2_r) \
+    X(r_O_3_i) \
+    X(r_O_3_r) \
+    X(r_O_4_i) \
+    X(r_O_4_r) \
+    X(r_O_5_i) \
+    X(r_O_5_r) \
+    X(r_O_6_i) \
+    X(r_O_6_r) \
+    X(r_O_7_i) \
+    X(r_O_7_r) \
+    X(r_O_8_i) \
+    X(r_O_8_r) \
+    X(r_O_9_r) \
+    X(c_O_1_c_O_1_c) \
+    X(c_O_2_c_O_2_c) \
+    X(c_O_3_c_O_3_c) \
+    X(c_O_4_c_O_4_c) \
+    X(c_O_5_c_O_5_c) \
+    X(c_O_6_c_O_6_c) \
+    X(c_O_7_c_O_7_c) \
+    X(c_O_8_c_O_8_c) \
+    X(c_O_9_c_O_9_c)
+
+#include "dengo_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* Drop-in entry points (rates_and_rate_tables.C.template:17-54, 149-274,
+   278-304, 309-350, 460-599; the reference's double &dt is double *dt). */
+oxygen_data *oxygen_setup_data(int *NumberOfFields, char ***FieldNames);
+void oxygen_read_rate_tables(oxygen_data *data);
+void oxygen_read_cooling_tables(oxygen_data *data);
+void oxygen_free_data(oxygen_data *data);
+double dengo_evolve_oxygen(double dtf, double *dt, double z, double *input,
+        double *rtol, double *atol, long long dims, oxygen_data *data);
+int calculate_rhs_oxygen(double *input, double *rhs, int nstrip, int nchem, void *sdata);
+int calculate_jacobian_oxygen(double *input, double *Joutput, int nstrip, int nchem, void *sdata);
+void oxygen_calculate_temperature(oxygen_data *data, double *input, int nstrip, int nchem);
+
+/* Backend extensions (no counterpart in the reference).
+   Allocate an empty struct; set tables from memory (tables[t * nT + i] in
+   struct order, T grid log-spaced on [T0, T1]); read the table blob from an
+   explicit path. */
+oxygen_data *oxygen_alloc_data(void);
+int oxygen_set_tables(oxygen_data *data, const double *tables, int ntables, int nT,
+        double T0, double T1);
+oxygen_data *oxygen_setup_data_from(const char *path, int *NumberOfFields, char ***FieldNames);
+/* Integrate dims cells held in device memory (AoS [dims][NSPECIES], updated
+   in place), enqueued on stream (a hipStream_t, NULL = default).  dt0 < 0
+   selects dtf/1e5.  With ttot_min non-NULL the call synchronises and returns
+   the smallest elapsed time and that cell's final dt. */
+int dengo_oxygen_evolve_device(oxygen_data *data, const dengo_options *opt, double dtf,
+        double dt0, double z, double *d_input, const double *d_rtol, double *d_atol,
+        long long dims, const dengo_cell_outputs *outputs, double *ttot_min,
+        double *dt_min, void *stream);
+/* Host-array form of dengo_evolve_oxygen with options and per-cell outputs. */
+double dengo_oxygen_evolve_host(oxygen_data *data, const dengo_options *opt, double dtf,
+        double *dt, double z, double *input, double *rtol, double *atol, long long dims,
+        int *status, int *nsub, int *nrej, int *nsweep, double *dt_cell, double *T_cell);
+/* Batched RHS f[dims][N], Jacobian J[dims][N][N] (column-major per cell, may
+   be NULL), temperature T[dims] and interpolated tables rates[dims][NTABLES]
+   (may be NULL) for number-density states on host memory; returns the number
+   of cells whose RHS saw a negative value, or a negative error code. */
+long long dengo_oxygen_eval(oxygen_data *data, const double *u, long long dims, double z,
+        double *f, double *J, double *T, double *rates);
+#ifdef __cplusplus
+}
+static inline double dengo_evolve_oxygen(double dtf, double &dt, double z, double *input,
+        double *rtol, double *atol, long long dims, oxygen_data *data)
+{
+    return dengo_evolve_oxygen(dtf, &dt, z, input, rtol, atol, dims, data);
+}
+#endif
+
+#endif

@@ -171,6 +171,8 @@ typedef struct primordial_data_ {
 } primordial_data;
 
 /* Table fields in struct order, for code that walks every table. */
+/* Field names in solver order, as <name>_setup_data reports them. */
+#define PRIMORDIAL_FIELD_NAMES {"H2I", "H2II", "HI", "HII", "HM", "HeI", "HeII", "HeIII", "de", "ge"}
 #define PRIMORDIAL_TABLE_FIELDS(X) \
     X(r_k01) \
     X(r_k02) \
@@ -218,9 +220,13 @@ typedef struct primordial_data_ {
     X(c_reHeIII_reHeIII) \
     X(c_reHII_reHII)
 
+#include "dengo_types.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
+/* Drop-in entry points (rates_and_rate_tables.C.template:17-54, 149-274,
+   278-304, 309-350, 460-599; the reference's double &dt is double *dt). */
 primordial_data *primordial_setup_data(int *NumberOfFields, char ***FieldNames);
 void primordial_read_rate_tables(primordial_data *data);
 void primordial_read_cooling_tables(primordial_data *data);
@@ -230,6 +236,33 @@ double dengo_evolve_primordial(double dtf, double *dt, double z, double *input,
 int calculate_rhs_primordial(double *input, double *rhs, int nstrip, int nchem, void *sdata);
 int calculate_jacobian_primordial(double *input, double *Joutput, int nstrip, int nchem, void *sdata);
 void primordial_calculate_temperature(primordial_data *data, double *input, int nstrip, int nchem);
+
+/* Backend extensions (no counterpart in the reference).
+   Allocate an empty struct; set tables from memory (tables[t * nT + i] in
+   struct order, T grid log-spaced on [T0, T1]); read the table blob from an
+   explicit path. */
+primordial_data *primordial_alloc_data(void);
+int primordial_set_tables(primordial_data *data, const double *tables, int ntables, int nT,
+        double T0, double T1);
+primordial_data *primordial_setup_data_from(const char *path, int *NumberOfFields, char ***FieldNames);
+/* Integrate dims cells held in device memory (AoS [dims][NSPECIES], updated
+   in place), enqueued on stream (a hipStream_t, NULL = default).  dt0 < 0
+   selects dtf/1e5.  With ttot_min non-NULL the call synchronises and returns
+   the smallest elapsed time and that cell's final dt. */
+int dengo_primordial_evolve_device(primordial_data *data, const dengo_options *opt, double dtf,
+        double dt0, double z, double *d_input, const double *d_rtol, double *d_atol,
+        long long dims, const dengo_cell_outputs *outputs, double *ttot_min,
+        double *dt_min, void *stream);
+/* Host-array form of dengo_evolve_primordial with options and per-cell outputs. */
+double dengo_primordial_evolve_host(primordial_data *data, const dengo_options *opt, double dtf,
+        double *dt, double z, double *input, double *rtol, double *atol, long long dims,
+        int *status, int *nsub, int *nrej, int *nsweep, double *dt_cell, double *T_cell);
+/* Batched RHS f[dims][N], Jacobian J[dims][N][N] (column-major per cell, may
+   be NULL), temperature T[dims] and interpolated tables rates[dims][NTABLES]
+   (may be NULL) for number-density states on host memory; returns the number
+   of cells whose RHS saw a negative value, or a negative error code. */
+long long dengo_primordial_eval(primordial_data *data, const double *u, long long dims, double z,
+        double *f, double *J, double *T, double *rates);
 #ifdef __cplusplus
 }
 static inline double dengo_evolve_primordial(double dtf, double &dt, double z, double *input,

@@ -272,6 +272,8 @@ void orc_prim_setup(orc_prim_data *d, const double *tables, int nT, double T0, d
     d->lb = log(d->bounds[0]);
     d->tables = tables;
     d->nT = nT;
+    d->n = ORC_PRIM_N;
+    d->ntab = ORC_PRIM_NTAB;
 }
 
 /* rates_and_rate_tables.C.template:309-350 with chemical_network.py:170-207:
@@ -312,7 +314,7 @@ void orc_prim_interpolate(orc_prim_data *d)
     d->bin_id = b;
     d->Tdef = (d->logT - t1) / (t2 - t1);
     d->dT = (t2 - t1);
-    for (int t = 0; t < ORC_PRIM_NTAB; ++t) {
+    for (int t = 0; t < d->ntab; ++t) {
         const double *r = d->tables + (size_t)t * d->nT;
         d->rs[t] = r[b] + d->Tdef * (r[b + 1] - r[b]);
     }
@@ -475,6 +477,131 @@ int orc_prim_jac(double *u, double *J, int nstrip, int nchem, void *sdata)
 }
 
 /* ------------------------------------------------------------------------ */
+/* Mass-action networks with linear cooling terms (ion-by-ion networks,     */
+/* dengo/ion_by_ion.py:33-55 with dengo/reaction_classes.py:182-197 for the  */
+/* ionisation/recombination pairs and :437-441 for the -c X de cooling).     */
+/* Same formulas as the primordial path: T from the weighted density and    */
+/* sum n/(gamma-1), rate law k * prod(n), energy row cooling/mdensity.       */
+/* ------------------------------------------------------------------------ */
+static void ma_temperature(orc_prim_data *d, const double *u)
+{
+    const orc_ma_net *net = d->net;
+    const int ge = net->n - 1;
+    const double kb = 1.3806504e-16, mh = 1.67e-24;
+    const double gamma = 5.e0 / 3.e0, gammaH2 = 7.e0 / 5.e0;
+    double density = 0.0, gsum = 0.0;
+    for (int s = 0; s < net->n; ++s)
+        if (s != ge && s != net->de) density += net->W[s] * u[s];
+    for (int s = 0; s < net->n; ++s)
+        if (s != ge) gsum += u[s] / ((net->h2gamma[s] ? gammaH2 : gamma) - 1.0);
+    double T = density * u[ge] * mh / (kb * gsum);
+    if (T < d->bounds[0]) T = d->bounds[0];
+    else if (T > d->bounds[1]) T = d->bounds[1];
+    d->T = T;
+    d->logT = (d->flags & ORC_FLAG_PORTABLE_LOG) ? orc_plog(T) : log(T);
+    d->invT = 1.0 / T;
+    d->dTs_ge = density * mh / (kb * gsum);
+}
+
+static double ma_net_change(const orc_rxn *r, int s)
+{
+    int up = 0, down = 0;
+    for (int k = 0; k < r->nprod; ++k) if (r->ps[k] == s) up += r->pc[k];
+    for (int k = 0; k < r->nslot; ++k) if (r->slot[k] == s) down += 1;
+    return (double)(up - down);
+}
+
+static double ma_mdensity(const orc_ma_net *net, const double *u)
+{
+    double m = 0.0;
+    for (int s = 0; s < net->n; ++s)
+        if (s != net->n - 1 && s != net->de) m += u[s];
+    return m * 1.67e-24;
+}
+
+static double ma_depow(double de, int p)
+{
+    return p == 0 ? 1.0 : (p == 1 ? de : de * de);
+}
+
+static int ma_rhs(double *u, double *f, orc_prim_data *d)
+{
+    const orc_ma_net *net = d->net;
+    const int n = net->n, ge = n - 1;
+    ma_temperature(d, u);
+    orc_prim_interpolate(d);
+    for (int s = 0; s < n; ++s)
+        if (u[s] < 0.0) return 1;
+    double md = ma_mdensity(net, u);
+    for (int s = 0; s < n; ++s) f[s] = 0.0;
+    for (int r = 0; r < net->nrxn; ++r) {
+        const orc_rxn *x = &net->rxn[r];
+        double flux = d->rs[x->table];
+        for (int k = 0; k < x->nslot; ++k) flux = flux * u[x->slot[k]];
+        for (int s = 0; s < ge; ++s) {
+            double nu = ma_net_change(x, s);
+            if (nu != 0.0) f[s] += nu * flux;
+        }
+    }
+    double sum = 0.0;
+    for (int c = 0; c < net->ncool; ++c) {
+        const orc_cool *t = &net->cool[c];
+        sum += t->sign * d->rs[t->table] * u[t->species] * ma_depow(u[net->de], t->depow);
+    }
+    f[ge] = sum / md;
+    return 0;
+}
+
+static int ma_jac(double *u, double *J, orc_prim_data *d)
+{
+    const orc_ma_net *net = d->net;
+    const int n = net->n, ge = n - 1, de = net->de;
+    for (int s = 0; s < n; ++s)
+        if (u[s] < 0.0) return 1;
+    double md = ma_mdensity(net, u);
+    for (int k = 0; k < n * n; ++k) J[k] = 0.0;
+    for (int r = 0; r < net->nrxn; ++r) {
+        const orc_rxn *x = &net->rxn[r];
+        for (int j = 0; j < x->nslot; ++j) {
+            double part = d->rs[x->table];
+            for (int k = 0; k < x->nslot; ++k)
+                if (k != j) part = part * u[x->slot[k]];
+            int col = x->slot[j];
+            for (int s = 0; s < ge; ++s) {
+                double nu = ma_net_change(x, s);
+                if (nu != 0.0) J[col * n + s] += nu * part;
+            }
+        }
+    }
+    double g[ORC_MAXN];
+    for (int k = 0; k < n; ++k) g[k] = 0.0;
+    for (int c = 0; c < net->ncool; ++c) {
+        const orc_cool *t = &net->cool[c];
+        double coef = t->sign * d->rs[t->table];
+        g[t->species] += coef * ma_depow(u[de], t->depow);
+        if (t->depow == 1) g[de] += coef * u[t->species];
+        else if (t->depow == 2) g[de] += 2.0 * coef * u[t->species] * u[de];
+    }
+    for (int col = 0; col < n; ++col) J[col * n + ge] = g[col] / md;
+    return 0;
+}
+
+static int any_rhs(double *u, double *f, int nstrip, int nchem, void *sdata)
+{
+    orc_prim_data *d = (orc_prim_data *)sdata;
+    if (!d->net) return orc_prim_rhs(u, f, nstrip, nchem, sdata);
+    d->rhs_calls++;
+    return ma_rhs(u, f, d);
+}
+
+static int any_jac(double *u, double *J, int nstrip, int nchem, void *sdata)
+{
+    orc_prim_data *d = (orc_prim_data *)sdata;
+    if (!d->net) return orc_prim_jac(u, J, nstrip, nchem, sdata);
+    return ma_jac(u, J, d);
+}
+
+/* ------------------------------------------------------------------------ */
 /* Newton / Gauss elimination (dengo/solvers/BE_chem_solve.C)                */
 /* ------------------------------------------------------------------------ */
 #define IDX(i, j, n) ((j) * (n) + (i))
@@ -595,21 +722,24 @@ static double evolve_cell(orc_be_f be, orc_prim_data *d, double dtf, double dt, 
                           double *in, double *rtol, double *atol, int input_is_number,
                           cell_stats *st)
 {
-    const int N = ORC_PRIM_N;
-    double scale[ORC_PRIM_N], prev[ORC_PRIM_N], u0[ORC_PRIM_N], s[ORC_PRIM_N],
-           gu[ORC_PRIM_N], Ju[ORC_PRIM_N * ORC_PRIM_N];
+    const int N = d->n;
+    const int ge = N - 1, de = d->net ? d->net->de : DE;
+    const double *Wt = d->net ? d->net->W : W;
+    const double *Fe = d->net ? d->net->free_e : FREE_E;
+    double scale[ORC_MAXN], prev[ORC_MAXN], u0[ORC_MAXN], s[ORC_MAXN], gu[ORC_MAXN],
+           Ju[ORC_MAXN * ORC_MAXN];
     if (!input_is_number) {
         for (int k = 0; k < N; ++k) {
-            if (k == GE) continue;
-            in[k] /= W[k] * 1.67e-24;
-            atol[k] /= W[k] * 1.67e-24;
+            if (k == ge) continue;
+            in[k] /= Wt[k] * 1.67e-24;
+            atol[k] /= Wt[k] * 1.67e-24;
         }
     }
     /* ensure_electron_consistency (:604-630) for one cell */
     double total_e = 0.0;
     for (int k = 0; k < N; ++k)
-        if (k != GE && k != DE) total_e += in[k] * FREE_E[k];
-    in[DE] = total_e;
+        if (k != ge && k != de) total_e += in[k] * Fe[k];
+    in[de] = total_e;
 
     if (dt < 0) dt = dtf / 1e5;
     d->current_z = z;
@@ -621,8 +751,7 @@ static double evolve_cell(orc_be_f be, orc_prim_data *d, double dtf, double dt, 
     st->nsub = 0;
     st->nrej = 0;
     while (ttot < dtf) {
-        int rv = be((orc_rhs_f)orc_prim_rhs, (orc_jac_f)orc_prim_jac, in, dt, rtol, atol,
-                    1, N, scale, (void *)d, u0, s, gu, Ju);
+        int rv = be(any_rhs, any_jac, in, dt, rtol, atol, 1, N, scale, (void *)d, u0, s, gu, Ju);
         for (int k = 0; k < N; ++k) {
             if (in[k] < 0) { rv = 1; break; }
         }
@@ -645,22 +774,22 @@ static double evolve_cell(orc_be_f be, orc_prim_data *d, double dtf, double dt, 
     st->dt = dt;
     if (!input_is_number) {
         for (int k = 0; k < N; ++k) {
-            if (k == GE) continue;
-            in[k] *= W[k] * 1.67e-24;
-            atol[k] *= W[k] * 1.67e-24;
+            if (k == ge) continue;
+            in[k] *= Wt[k] * 1.67e-24;
+            atol[k] *= Wt[k] * 1.67e-24;
         }
     }
     return ttot;
 }
 
-double orc_prim_evolve(double dtf, double *dt, double z, double *input,
-                       double *rtol, double *atol, long long dims,
-                       const double *tables, int nT, double T0, double T1,
-                       int input_is_number, int *status, int *nsub, int *nrej,
-                       int *nsweep, double *dt_out, double *T_out,
-                       orc_be_f be, int nthreads, int flags)
+static double evolve_all(const orc_ma_net *net, double dtf, double *dt, double z, double *input,
+                         double *rtol, double *atol, long long dims, const double *tables,
+                         int nT, double T0, double T1, int input_is_number, int *status,
+                         int *nsub, int *nrej, int *nsweep, double *dt_out, double *T_out,
+                         orc_be_f be, int nthreads, int flags)
 {
     const double dt_in = *dt;
+    const int N = net ? net->n : ORC_PRIM_N;
     double tmin = 1e300, dtmin = dt_in;
     long long imin = -1;
     if (be == NULL) be = orc_BE_chem_solve;
@@ -673,9 +802,13 @@ double orc_prim_evolve(double dtf, double *dt, double z, double *input,
         cell_stats st;
         orc_prim_setup(&d, tables, nT, T0, T1);
         d.flags = flags;
-        double t = evolve_cell(be, &d, dtf, dt_in, z, input + c * ORC_PRIM_N,
-                               rtol + c * ORC_PRIM_N, atol + c * ORC_PRIM_N,
-                               input_is_number, &st);
+        if (net) {
+            d.net = net;
+            d.n = net->n;
+            d.ntab = net->ntab;
+        }
+        double t = evolve_cell(be, &d, dtf, dt_in, z, input + c * N, rtol + c * N,
+                               atol + c * N, input_is_number, &st);
         if (status) status[c] = st.status;
         if (nsub) nsub[c] = st.nsub;
         if (nrej) nrej[c] = st.nrej;
@@ -698,6 +831,53 @@ double orc_prim_evolve(double dtf, double *dt, double z, double *input,
     return dims > 0 ? tmin : 0.0;
 }
 
+double orc_ma_evolve(const orc_ma_net *net, double dtf, double *dt, double z, double *input,
+                     double *rtol, double *atol, long long dims, const double *tables,
+                     int nT, double T0, double T1, int input_is_number, int *status,
+                     int *nsub, int *nrej, int *nsweep, double *dt_out, double *T_out,
+                     orc_be_f be, int nthreads, int flags)
+{
+    return evolve_all(net, dtf, dt, z, input, rtol, atol, dims, tables, nT, T0, T1,
+                      input_is_number, status, nsub, nrej, nsweep, dt_out, T_out, be, nthreads,
+                      flags);
+}
+
+int orc_ma_eval(const orc_ma_net *net, const double *u, long long dims, const double *tables,
+                int nT, double T0, double T1, double *f, double *J, double *T,
+                double *rates, int flags)
+{
+    const int n = net->n;
+    int nneg = 0;
+    for (long long c = 0; c < dims; ++c) {
+        orc_prim_data d;
+        double uc[ORC_MAXN];
+        orc_prim_setup(&d, tables, nT, T0, T1);
+        d.flags = flags;
+        d.net = net;
+        d.n = n;
+        d.ntab = net->ntab;
+        memcpy(uc, u + c * n, sizeof(double) * n);
+        int rv = any_rhs(uc, f + c * n, 1, n, &d);
+        if (rv) nneg++;
+        else if (J) any_jac(uc, J + c * n * n, 1, n, &d);
+        if (T) T[c] = d.T;
+        if (rates) memcpy(rates + c * net->ntab, d.rs, sizeof(double) * net->ntab);
+    }
+    return nneg;
+}
+
+double orc_prim_evolve(double dtf, double *dt, double z, double *input,
+                       double *rtol, double *atol, long long dims,
+                       const double *tables, int nT, double T0, double T1,
+                       int input_is_number, int *status, int *nsub, int *nrej,
+                       int *nsweep, double *dt_out, double *T_out,
+                       orc_be_f be, int nthreads, int flags)
+{
+    return evolve_all(NULL, dtf, dt, z, input, rtol, atol, dims, tables, nT, T0, T1,
+                      input_is_number, status, nsub, nrej, nsweep, dt_out, T_out, be, nthreads,
+                      flags);
+}
+
 int orc_prim_eval(const double *u, long long dims, const double *tables, int nT,
                   double T0, double T1, double z, double *f, double *J,
                   double *T, double *rates, int flags)
@@ -714,7 +894,7 @@ int orc_prim_eval(const double *u, long long dims, const double *tables, int nT,
         if (rv) nneg++;
         else if (J) orc_prim_jac(uc, J + c * ORC_PRIM_N * ORC_PRIM_N, 1, ORC_PRIM_N, &d);
         if (T) T[c] = d.T;
-        if (rates) memcpy(rates + c * ORC_PRIM_NTAB, d.rs, sizeof(d.rs));
+        if (rates) memcpy(rates + c * ORC_PRIM_NTAB, d.rs, sizeof(double) * ORC_PRIM_NTAB);
     }
     return nneg;
 }

@@ -42,6 +42,27 @@ int orc_BE_chem_solve(orc_rhs_f f, orc_jac_f J, double *u, double dt, double *rt
 /* Gauss_Elim, restated (BE_chem_solve.C:329-382). */
 int orc_gauss_elim(double *A, double *x, double *b, int n);
 
+/* Generic mass-action network with linear cooling terms (ion-by-ion
+ * networks): reaction rate law k * prod(n_slot), cooling term
+ * sign * c * n_species * de^depow.  The energy unknown is last. */
+#define ORC_MAXN 16
+#define ORC_MAXTAB 64
+#define ORC_MAXRXN 64
+typedef struct {
+    int table, nslot, slot[3], nprod, pc[2], ps[2];
+} orc_rxn;
+typedef struct {
+    int table, species, depow;
+    double sign;
+} orc_cool;
+typedef struct orc_ma_net {
+    int n, ntab, nrxn, ncool, de;      /* energy index is n - 1 */
+    double W[ORC_MAXN], free_e[ORC_MAXN];
+    int h2gamma[ORC_MAXN];             /* 1: gamma = 7/5, else 5/3 */
+    orc_rxn rxn[ORC_MAXRXN];
+    orc_cool cool[ORC_MAXTAB];
+} orc_ma_net;
+
 /* Per-call state handed to the RHS/Jacobian callbacks (nstrip must be 1). */
 typedef struct orc_prim_data {
     double bounds[2];
@@ -53,9 +74,11 @@ typedef struct orc_prim_data {
     /* last temperature / interpolation state (from the last RHS call) */
     double T, logT, invT, dTs_ge, Tdef, dT;
     int bin_id;
-    double rs[ORC_PRIM_NTAB];
+    double rs[ORC_MAXTAB];
     long long rhs_calls;    /* one per Newton sweep */
     int flags;              /* ORC_FLAG_* */
+    const orc_ma_net *net;  /* NULL: the primordial network */
+    int n, ntab;
 } orc_prim_data;
 
 /* Accepted-substep cap (default 50000, as the reference). */
@@ -93,6 +116,16 @@ double orc_prim_evolve(double dtf, double *dt, double z, double *input,
 int orc_prim_eval(const double *u, long long dims, const double *tables, int nT,
                   double T0, double T1, double z, double *f, double *J,
                   double *T, double *rates, int flags);
+
+/* Mass-action network counterparts of orc_prim_evolve / orc_prim_eval. */
+double orc_ma_evolve(const orc_ma_net *net, double dtf, double *dt, double z, double *input,
+                     double *rtol, double *atol, long long dims, const double *tables,
+                     int nT, double T0, double T1, int input_is_number, int *status,
+                     int *nsub, int *nrej, int *nsweep, double *dt_out, double *T_out,
+                     orc_be_f be, int nthreads, int flags);
+int orc_ma_eval(const orc_ma_net *net, const double *u, long long dims, const double *tables,
+                int nT, double T0, double T1, double *f, double *J, double *T,
+                double *rates, int flags);
 
 #ifdef __cplusplus
 }

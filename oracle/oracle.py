@@ -125,3 +125,112 @@ def prim_evolve(inp, rtol, atol, tables, dtf, dt=-1.0, z=-1.0, T_bounds=(1.0, 1e
         be, nthreads, flags)
     return dict(state=out, atol=atol_o, ttot=ttot, dt=dtc.value, status=status,
                 nsub=nsub, nrej=nrej, nsweep=nsweep, dt_cell=dt_out, T=T_out)
+
+
+# ---------------------------------------------------------------------------
+# Ion-by-ion networks (dengo/ion_by_ion.py:33-55): ions X_1..X_{Z+1},
+# ionisation X_k + e -> X_{k+1} + 2e (dengo/reaction_classes.py:182-186),
+# recombination X_k + e -> X_{k-1} (:193-197), cooling -c_k X_k de (:437-441).
+# Tables in the struct order: reactions by name (case-insensitive), then
+# the cooling tables.
+# ---------------------------------------------------------------------------
+class _Rxn(ctypes.Structure):
+    _fields_ = [("table", ctypes.c_int), ("nslot", ctypes.c_int), ("slot", ctypes.c_int * 3),
+                ("nprod", ctypes.c_int), ("pc", ctypes.c_int * 2), ("ps", ctypes.c_int * 2)]
+
+
+class _Cool(ctypes.Structure):
+    _fields_ = [("table", ctypes.c_int), ("species", ctypes.c_int), ("depow", ctypes.c_int),
+                ("sign", ctypes.c_double)]
+
+
+class MANet(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("ntab", ctypes.c_int), ("nrxn", ctypes.c_int),
+                ("ncool", ctypes.c_int), ("de", ctypes.c_int),
+                ("W", ctypes.c_double * 16), ("free_e", ctypes.c_double * 16),
+                ("h2gamma", ctypes.c_int * 16), ("rxn", _Rxn * 64), ("cool", _Cool * 64)]
+
+
+def ion_chain_net(Z, weight):
+    """orc_ma_net for an element with atomic number Z (species X_1..X_{Z+1},
+    de, ge)."""
+    net = MANet()
+    nion = Z + 1
+    de = nion
+    net.n, net.de = nion + 2, de
+    for k in range(nion):
+        net.W[k] = weight
+        net.free_e[k] = float(k)
+    net.W[de] = net.W[de + 1] = 1.0
+    names = []
+    for k in range(nion):
+        if k + 1 < nion:
+            names.append(("X_%d_i" % (k + 1), k, "i"))
+        if k - 1 >= 0:
+            names.append(("X_%d_r" % (k + 1), k, "r"))
+    names.sort(key=lambda t: t[0].lower())
+    for t, (_, k, kind) in enumerate(names):
+        r = net.rxn[t]
+        r.table, r.nslot = t, 2
+        r.slot[0], r.slot[1] = k, de
+        if kind == "i":
+            r.nprod = 2
+            r.pc[0], r.ps[0], r.pc[1], r.ps[1] = 1, k + 1, 2, de
+        else:
+            r.nprod = 1
+            r.pc[0], r.ps[0] = 1, k - 1
+    net.nrxn = len(names)
+    for k in range(nion):
+        c = net.cool[k]
+        c.table, c.species, c.depow, c.sign = len(names) + k, k, 1, -1.0
+    net.ncool = nion
+    net.ntab = len(names) + nion
+    return net
+
+
+def _ma_lib():
+    lib = load()
+    if not getattr(lib, "_ma_ready", False):
+        lib.orc_ma_evolve.restype = ctypes.c_double
+        lib.orc_ma_evolve.argtypes = [ctypes.POINTER(MANet)] + lib.orc_prim_evolve.argtypes
+        lib.orc_ma_eval.restype = ctypes.c_int
+        lib.orc_ma_eval.argtypes = [ctypes.POINTER(MANet), _dp, ctypes.c_longlong, _dp,
+                                    ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp,
+                                    _dp, _dp, ctypes.c_int]
+        lib._ma_ready = True
+    return lib
+
+
+def ma_eval(net, u, tables, T_bounds, flags=0):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    n, N = u.shape
+    f = np.zeros((n, N))
+    J = np.zeros((n, N, N))
+    T = np.zeros(n)
+    rates = np.zeros((n, net.ntab))
+    tab = np.ascontiguousarray(tables, dtype=np.float64)
+    nneg = _ma_lib().orc_ma_eval(ctypes.byref(net), _ptr(u), n, _ptr(tab), tab.shape[1],
+                                 T_bounds[0], T_bounds[1], _ptr(f), _ptr(J), _ptr(T),
+                                 _ptr(rates), flags)
+    return f, J, T, rates, nneg
+
+
+def ma_evolve(net, inp, rtol, atol, tables, dtf, T_bounds, dt=-1.0, z=-1.0,
+              input_is_number=False, nthreads=0, flags=0, max_substeps=50000):
+    lib = _ma_lib()
+    lib.orc_set_max_substeps(max_substeps)
+    out = np.array(inp, dtype=np.float64, copy=True, order="C")
+    atol_o = np.array(atol, dtype=np.float64, copy=True, order="C")
+    rtol_c = np.array(rtol, dtype=np.float64, copy=True, order="C")
+    n = out.shape[0]
+    status, nsub, nrej, nsweep = (np.zeros(n, dtype=np.int32) for _ in range(4))
+    dt_out, T_out = np.zeros(n), np.zeros(n)
+    tab = np.ascontiguousarray(tables, dtype=np.float64)
+    dtc = ctypes.c_double(dt)
+    ttot = lib.orc_ma_evolve(
+        ctypes.byref(net), dtf, ctypes.byref(dtc), z, _ptr(out), _ptr(rtol_c), _ptr(atol_o), n,
+        _ptr(tab), tab.shape[1], T_bounds[0], T_bounds[1], int(input_is_number),
+        _ptr(status, _ip), _ptr(nsub, _ip), _ptr(nrej, _ip), _ptr(nsweep, _ip), _ptr(dt_out),
+        _ptr(T_out), None, nthreads, flags)
+    return dict(state=out, atol=atol_o, ttot=ttot, dt=dtc.value, status=status, nsub=nsub,
+                nrej=nrej, nsweep=nsweep, dt_cell=dt_out, T=T_out)

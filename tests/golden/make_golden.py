@@ -3,7 +3,9 @@
 Writes tests/golden/primordial_cells.npz: 96 seeded cells (config-1
 distributions, seed 11) with inputs, tolerances and the oracle's outputs
 (final state, atol, elapsed time, per-cell counters) for ln T taken with libm
-("libm_*") and with the portable log the device code uses ("plog_*").
+("libm_*") and with the portable log the device code uses ("plog_*"), and
+tests/golden/oxygen_cells.npz: the same for 128 O I-IX cells (seed 12) of the
+synthetic oxygen network (table seed 3).
 dtf = 1e12 s, dt = -1, z = -1, as the emitted main runs the solver.
 """
 import os
@@ -17,7 +19,8 @@ sys.path[:0] = [ROOT, os.path.join(ROOT, "oracle")]
 
 import oracle  # noqa: E402
 from dengo_amd.ics import primordial_ics, tolerances  # noqa: E402
-from dengo_amd.networks import primordial_tables  # noqa: E402
+from dengo_amd.ics import oxygen_ics  # noqa: E402
+from dengo_amd.networks import oxygen_tables, primordial_tables  # noqa: E402
 
 
 def main():
@@ -33,6 +36,20 @@ def main():
         out["%s_dt" % tag] = r["dt"]
     np.savez_compressed(os.path.join(HERE, "primordial_cells.npz"), **out)
     print("wrote primordial_cells.npz; capped cells:", int((out["libm_status"] == 2).sum()))
+
+    _, otab = oxygen_tables()
+    net = oracle.ion_chain_net(8, 15.9994)
+    inp, _, _ = oxygen_ics(128, seed=12)
+    rtol, atol = tolerances(inp)
+    out = dict(input=inp, rtol=rtol, atol=atol, dtf=1e12, seed=12)
+    for tag, flags in (("libm", 0), ("plog", oracle.PORTABLE_LOG)):
+        r = oracle.ma_evolve(net, inp, rtol, atol, otab, 1e12, (1e4, 1e9), flags=flags)
+        for k in ("state", "atol", "status", "nsub", "nrej", "nsweep", "dt_cell", "T"):
+            out["%s_%s" % (tag, k)] = r[k]
+        out["%s_ttot" % tag] = r["ttot"]
+        out["%s_dt" % tag] = r["dt"]
+    np.savez_compressed(os.path.join(HERE, "oxygen_cells.npz"), **out)
+    print("wrote oxygen_cells.npz")
 
 
 if __name__ == "__main__":

@@ -1,0 +1,78 @@
+"""Pins on the oracle's mass-action path (ion-by-ion networks), no GPU.
+
+The Newton/BE/substep driver is shared with the primordial path (pinned
+against the reference's own BE_chem_solve.C in test_oracle.py); these tests
+pin the ion-chain RHS and Jacobian: conservation of O nuclei and of charge,
+finite differences, the temperature inversion, and the fixture."""
+import numpy as np
+
+import oracle
+from dengo_amd.ics import oxygen_ics, tolerances
+from dengo_amd.networks import oxygen_tables
+
+NET = oracle.ion_chain_net(8, 15.9994)
+TB = (1e4, 1e9)
+
+
+def test_ion_rhs_conservation_and_temperature():
+    _, tab = oxygen_tables()
+    _, n, T = oxygen_ics(300, seed=8)
+    f, J, T2, rates, neg = oracle.ma_eval(NET, n, tab, TB)
+    assert neg == 0
+    mag = np.abs(f[:, :10]).max(axis=1)
+    assert np.all(np.abs(f[:, :9].sum(axis=1)) <= 1e-12 * mag)
+    charge = (f[:, :9] * np.arange(9)).sum(axis=1) - f[:, 9]
+    assert np.all(np.abs(charge) <= 1e-12 * mag)
+    inside = (T > 1e4) & (T < 1e9)
+    assert np.allclose(T2[inside], T[inside], rtol=1e-12)
+    # electrons recombine onto ion k at rate k_r: the ion chain's rate law
+    k2r = rates[:, 2]        # table 2 is O_2_r (O_2 + e -> O_1)
+    k1i = rates[:, 0]        # table 0 is O_1_i (O_1 + e -> O_2 + 2e)
+    assert np.allclose(f[:, 0], -k1i * n[:, 0] * n[:, 9] + k2r * n[:, 1] * n[:, 9],
+                       rtol=1e-9, atol=1e-12 * mag.max())
+
+
+def test_ion_jacobian_finite_differences():
+    from dengo_amd.ics import ion_temperature_ge
+    _, tab = oxygen_tables()
+    _, n, T = oxygen_ics(64, seed=9)
+    f0, J, T0, _, _ = oracle.ma_eval(NET, n, tab, TB)
+
+    def keep_T(u):
+        v = u.copy()
+        v[:, 10] = ion_temperature_ge(v, T0, 15.9994, 10)
+        return v
+
+    def md(u):
+        return 1.67e-24 * u[:, :9].sum(axis=1)
+
+    for col in range(10):
+        h = 1e-6 * n[:, col]
+        up, dn = n.copy(), n.copy()
+        up[:, col] += h
+        dn[:, col] -= h
+        up, dn = keep_T(up), keep_T(dn)
+        fu = oracle.ma_eval(NET, up, tab, TB)[0]
+        fd = oracle.ma_eval(NET, dn, tab, TB)[0]
+        fu[:, 10] *= md(up)
+        fd[:, 10] *= md(dn)
+        fdj = (fu - fd) / (2 * h[:, None])
+        an = J[:, col, :].copy()
+        an[:, 10] *= md(n)
+        fm = f0.copy()
+        fm[:, 10] *= md(n)
+        # rows are sums of opposing ionisation/recombination terms: measure the
+        # error against |f|/n_col (the size of the terms differenced) + |J|
+        scale = np.abs(an) + np.abs(fm) / n[:, col:col + 1] + 1e-300
+        assert (np.abs(fdj - an) / scale).max() < 1e-5, col
+    assert np.all(J[:, 10, :] == 0.0)
+
+
+def test_ion_golden_fixture():
+    _, tab = oxygen_tables()
+    g = np.load("tests/golden/oxygen_cells.npz")
+    for tag, flags in (("libm", 0), ("plog", oracle.PORTABLE_LOG)):
+        r = oracle.ma_evolve(NET, g["input"], g["rtol"], g["atol"], tab, float(g["dtf"]), TB,
+                             flags=flags)
+        for k in ("state", "atol", "status", "nsub", "nrej", "nsweep"):
+            assert np.array_equal(r[k], g["%s_%s" % (tag, k)]), (tag, k)
