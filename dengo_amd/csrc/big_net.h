@@ -1,0 +1,752 @@
+// Dense-Jacobian BE solver for large networks (N = 128 unknowns per cell):
+// one 512-thread workgroup (eight waves) per cell.
+//
+// The reference solver (dengo/solvers/BE_chem_solve.C:79-382 driven by
+// dengo/templates/rates_and_rate_tables.C.template:174-257) is restated here
+// for an isothermal mass-action network whose reactions are runtime data:
+// species X_0..X_{N-1}, reactions a + b -> c + d, rates
+// k = A exp(beta ln(T/300) - gamma/T) per cell.
+//
+// Layout of one cell on the CU:
+//   * the N x N Newton matrix lives in registers, cut into eight 16-column
+//     tile columns; wave w owns tile column w and lane l of a wave holds rows
+//     l and l + 64 of it (32 FP64 values per lane);
+//   * rates, fluxes, the state vectors, the pivot bookkeeping and one 16-column
+//     chunk of the Jacobian (or one panel's multipliers) at a time live in LDS.
+// The RHS and the Jacobian are gathered, not scattered: a host-built program
+// gives every thread a list of (entry, reaction, partner, stoichiometric
+// change) operations in the reference's summation order, so every entry is
+// one thread's ordered sum (no atomics, deterministic).
+//
+// Elimination is blocked by tile column (a "panel" of 16 pivot steps): the
+// wave owning the panel factors it alone - pivot search by wave reductions
+// (DPP), pivot row by lane reads, no workgroup barrier per step - and writes
+// the 16 multiplier columns to LDS; then every wave applies the panel's 16
+// steps, in order, to its own columns right of the panel.  Every matrix
+// element receives exactly the updates of the unblocked algorithm in the same
+// order, so the faithful mode stays bit-identical to the reference.
+//
+// Partial pivoting is virtual: rows never move.  pos[row] is the row's current
+// position in the reference's swapped order, so the pivot search reproduces
+// the reference's rule (first maximum of |a| in the swapped order,
+// BE_chem_solve.C:340-347) exactly, NaN entries included.
+//   kLU  reference-faithful: same multipliers (true division), separately
+//        rounded products and differences, sequential row-oriented back
+//        substitution (:372-380); bit-identical to a plain C build.
+//   kGJ  Gauss-Jordan: each pivot column is eliminated from every other row,
+//        so the solve needs no back substitution; reciprocal pivots and FMA.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "dengo_device.h"
+#include "dengo_types.h"
+
+namespace dengo {
+namespace big {
+
+constexpr int N = 128;
+constexpr int TILE = 16;
+constexpr int NT = N / TILE;           // tile columns
+constexpr int NWAVE = 8;               // waves per cell
+constexpr int NTHR = 64 * NWAVE;       // threads per cell
+constexpr int TCW = NT / NWAVE;        // tile columns per wave
+constexpr int RPL = N / 64;            // rows per lane
+constexpr int CHUNK_LD = N + 1;        // LDS column stride of a Jacobian chunk
+constexpr int CHUNK_SZ = TILE * CHUNK_LD;
+constexpr int MAX_RXN = 2048;
+constexpr int DUMMY_SPECIES = N;
+constexpr int OPS_BATCH = 8;           // program lengths are padded to this
+constexpr int BUF_SZ = MAX_RXN + 32;
+
+static_assert(CHUNK_SZ <= BUF_SZ && TILE * N <= BUF_SZ, "shared buffer size");
+
+enum { kLU = 0, kGJ = 1 };
+
+// RHS operation: bits 0-7 species (N: padding), bits 8-10 change + 4,
+// bits 11-31 reaction.
+DENGO_HD unsigned rhs_op(int s, int nu, int r)
+{
+    return (unsigned)s | ((unsigned)(nu + 4) << 8) | ((unsigned)r << 11);
+}
+// Jacobian operation for column chunk entry (row, lc): bits 0-6 row, 7-10 lc,
+// 11-13 change + 4 (4: no contribution, only the entry's zero is written),
+// 14-20 partner species (the other reactant), 21-31 reaction.
+DENGO_HD unsigned jac_op(int row, int lc, int nu, int partner, int r)
+{
+    return (unsigned)row | ((unsigned)lc << 7) | ((unsigned)(nu + 4) << 11) |
+           ((unsigned)partner << 14) | ((unsigned)r << 21);
+}
+
+struct NetDev {
+    int nrxn;
+    const int4 *rxn;                   // (a, b, c, d)
+    const double *A, *beta, *gamma;
+    const unsigned *rhs_ops;           // [rhs_len][NTHR]
+    int rhs_len;
+    const unsigned *jac_ops;           // chunk J: [jac_len[J]][NTHR] at jac_off[J] * NTHR
+    int jac_len[NT];
+    int jac_off[NT];
+};
+
+struct Shared {
+    double k[MAX_RXN];
+    double buf[BUF_SZ];                // fluxes; a Jacobian chunk; a panel's multipliers
+    double u[N], u0[N], sc[N], inv[N], prev[N], at[N], rt[N], x[N], s[N], pv[N];
+    double f[N + 1];
+    unsigned short ab[MAX_RXN];        // a | b << 8
+    int pos[N], prow[N];
+    unsigned long long ball[2][4];
+    int anyw[2][NWAVE];
+    long long cell;
+};
+
+template <int V>
+using IC = std::integral_constant<int, V>;
+
+// Calls f(IC<i>) for the runtime value i in [I, E); i must be uniform.  The
+// empty asm statement differs per case, so the compiler cannot merge the
+// cases into one block that indexes the register array at run time (which
+// would move the array to scratch memory).
+template <int I, int E, class F>
+__device__ __forceinline__ void dispatch(int i, F &&f)
+{
+    if constexpr (I < E) {
+        if (i == I) {
+            f(IC<I>{});
+            asm volatile("; case %0" ::"n"(I));
+        } else {
+            dispatch<I + 1, E>(i, f);
+        }
+    }
+}
+
+struct Geo {
+    int t, lane, w;
+    __device__ Geo()
+    {
+        t = threadIdx.x;
+        lane = t & 63;
+        w = t >> 6;
+    }
+    __device__ int row(int h) const { return lane + 64 * h; }
+};
+
+// Wave-wide maximum / minimum of a 32-bit value: prefix steps row_shr 1, 2,
+// 4, 8 inside each row of 16 lanes, then row_bcast 15 and 31 carry the row
+// results up to lane 63, which is read back.
+template <bool MAX>
+__device__ __forceinline__ unsigned wave_reduce(unsigned v)
+{
+    constexpr int id = MAX ? 0 : -1;
+    auto op = [](unsigned a, unsigned b) { return MAX ? (a > b ? a : b) : (a < b ? a : b); };
+    v = op(v, (unsigned)__builtin_amdgcn_update_dpp(id, (int)v, 0x111, 0xf, 0xf, false));
+    v = op(v, (unsigned)__builtin_amdgcn_update_dpp(id, (int)v, 0x112, 0xf, 0xf, false));
+    v = op(v, (unsigned)__builtin_amdgcn_update_dpp(id, (int)v, 0x114, 0xf, 0xf, false));
+    v = op(v, (unsigned)__builtin_amdgcn_update_dpp(id, (int)v, 0x118, 0xf, 0xf, false));
+    v = op(v, (unsigned)__builtin_amdgcn_update_dpp(id, (int)v, 0x142, 0xa, 0xf, false));
+    v = op(v, (unsigned)__builtin_amdgcn_update_dpp(id, (int)v, 0x143, 0xc, 0xf, false));
+    return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+__device__ __forceinline__ double readlane_d(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Block-wide OR with one barrier (two alternating flag sets).
+__device__ __forceinline__ bool block_any(bool p, Shared &S, const Geo &G, int &parity)
+{
+    const unsigned long long b = __ballot(p);
+    if (G.lane == 0) S.anyw[parity][G.w] = b != 0ULL;
+    __syncthreads();
+    bool r = false;
+#pragma unroll
+    for (int q = 0; q < NWAVE; ++q) r = r || S.anyw[parity][q] != 0;
+    parity ^= 1;
+    return r;
+}
+
+// Loads operation batch q (OPS_BATCH entries) of thread t's program in a
+// [len][NTHR] layout at the uniform address base (32-bit lane offsets).
+__device__ __forceinline__ void load_ops(const unsigned *base, unsigned t, int q,
+                                         unsigned (&o)[OPS_BATCH])
+{
+#pragma unroll
+    for (int i = 0; i < OPS_BATCH; ++i) o[i] = base[t + (unsigned)(q + i) * NTHR];
+}
+
+// ---------------------------------------------------------------------------
+// RHS: fluxes (k * u_a) * u_b, then f_s = sum over reactions in ascending
+// order of nu * flux (the reference's mass-action sum).  u = unscaled state.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void run_rhs(const NetDev &net, Shared &S, const Geo &G)
+{
+    const unsigned *ops = net.rhs_ops;
+    const unsigned t = G.t;
+    const int L = net.rhs_len;
+    unsigned cur[OPS_BATCH], nxt[OPS_BATCH];
+    if (L > 0) load_ops(ops, t, 0, cur);
+    for (int r = G.t; r < net.nrxn; r += NTHR) {
+        const int ab = S.ab[r];
+        double fl = S.k[r];
+        fl = fl * S.u[ab & 0xff];
+        fl = fl * S.u[ab >> 8];
+        S.buf[r] = fl;
+    }
+    __syncthreads();
+    double acc = 0.0;
+    int e_cur = -1;
+    for (int q = 0; q < L; q += OPS_BATCH) {
+        if (q + OPS_BATCH < L) load_ops(ops, t, q + OPS_BATCH, nxt);
+#pragma unroll
+        for (int i = 0; i < OPS_BATCH; ++i) {
+            const unsigned o = cur[i];
+            const int e = o & 0xff, nu = (int)((o >> 8) & 7) - 4, r = o >> 11;
+            if (e != e_cur) {
+                if (e_cur >= 0) S.f[e_cur] = acc;
+                acc = 0.0;
+                e_cur = e;
+            }
+            if (nu != 0) acc = acc + (double)nu * S.buf[r];
+        }
+#pragma unroll
+        for (int i = 0; i < OPS_BATCH; ++i) cur[i] = nxt[i];
+    }
+    if (e_cur >= 0) S.f[e_cur] = acc;
+}
+
+// Jacobian column chunk J (columns 16J..16J+15) of d f / d u into S.buf[lc *
+// CHUNK_LD + row]: entry = sum over (reaction, reactant slot) in the
+// reference's order of nu * (k * u_partner).  u = unscaled state.
+__device__ __forceinline__ void run_jac_chunk(const NetDev &net, Shared &S, const Geo &G, int J)
+{
+    const unsigned *ops = net.jac_ops + (size_t)net.jac_off[J] * NTHR;
+    const unsigned t = G.t;
+    const int L = net.jac_len[J];
+    unsigned cur[OPS_BATCH], nxt[OPS_BATCH];
+    if (L > 0) load_ops(ops, t, 0, cur);
+    double acc = 0.0;
+    int e_cur = -1;
+    for (int q = 0; q < L; q += OPS_BATCH) {
+        if (q + OPS_BATCH < L) load_ops(ops, t, q + OPS_BATCH, nxt);
+#pragma unroll
+        for (int i = 0; i < OPS_BATCH; ++i) {
+            const unsigned o = cur[i];
+            const int e = (int)((o >> 7) & 15) * CHUNK_LD + (int)(o & 127);
+            const int nu = (int)((o >> 11) & 7) - 4, partner = (o >> 14) & 127, r = o >> 21;
+            if (e != e_cur) {
+                if (e_cur >= 0) S.buf[e_cur] = acc;
+                acc = 0.0;
+                e_cur = e;
+            }
+            if (nu != 0) {
+                const double part = S.k[r] * S.u[partner];
+                acc = acc + (double)nu * part;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < OPS_BATCH; ++i) cur[i] = nxt[i];
+    }
+    if (e_cur >= 0) S.buf[e_cur] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Dense solve of M s = x (S.x indexed by physical row); result in S.s.
+// ---------------------------------------------------------------------------
+// U rows of the faithful mode, packed: row k holds columns k+1..N-1 from
+// offset k (N - 1) - k (k - 1) / 2.
+constexpr int U_PACKED = N * (N - 1) / 2;
+__device__ __forceinline__ int urow_off(int k) { return k * (N - 1) - (k * (k - 1)) / 2; }
+
+// Values of row p (held by lane p % 64, slot p / 64) of a tile column, read
+// into uniform registers.
+template <int TC>
+__device__ __forceinline__ void pivot_row(const double (&M)[TCW][RPL][TILE], int p,
+                                          double (&u)[TILE])
+{
+    const int lp = p & 63;
+    if (p >> 6) {
+#pragma unroll
+        for (int c = 0; c < TILE; ++c) u[c] = readlane_d(M[TC][1][c], lp);
+    } else {
+#pragma unroll
+        for (int c = 0; c < TILE; ++c) u[c] = readlane_d(M[TC][0][c], lp);
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ void row_update(double &m, double l, double u)
+{
+    if constexpr (MODE == kLU) {
+        const double t = l * u;
+        m = m - t;
+    } else {
+        m = fma(-l, u, m);
+    }
+}
+
+// Factors panel P (tile column P, slot TC of the calling wave): 16 pivot steps.
+template <int MODE, int TC>
+__device__ __forceinline__ void factor_panel(double (&M)[TCW][RPL][TILE], Shared &S, const Geo &G,
+                                             int P, double *Ul)
+{
+    int pos[RPL];
+    double xr[RPL];
+#pragma unroll
+    for (int h = 0; h < RPL; ++h) {
+        pos[h] = S.pos[G.row(h)];
+        xr[h] = S.x[G.row(h)];
+    }
+#pragma unroll 1
+    for (int kk = 0; kk < TILE; ++kk) {
+        const int k = TILE * P + kk;
+        double v[RPL];
+        dispatch<0, TILE>(kk, [&](auto KK) {
+#pragma unroll
+            for (int h = 0; h < RPL; ++h) v[h] = M[TC][h][KK.value];
+        });
+        // pivot search: largest |a| among rows not yet pivoted, first in the
+        // swapped order on ties; a NaN wins only at position k (where the
+        // reference's scan starts), elsewhere it never compares larger.
+        // Fast path: the top 32 bits of |a| single out one row.
+        unsigned long long key[RPL];
+        bool cand[RPL];
+        unsigned lhi = 0u;
+#pragma unroll
+        for (int h = 0; h < RPL; ++h) {
+            cand[h] = pos[h] >= k;
+            unsigned long long kb = (unsigned long long)__double_as_longlong(fabs(v[h]));
+            if (v[h] != v[h]) kb = pos[h] == k ? 0x7ff0000000000000ULL : 0ULL;
+            key[h] = cand[h] ? kb : 0ULL;
+            const unsigned hi = (unsigned)(key[h] >> 32);
+            lhi = hi > lhi ? hi : lhi;
+        }
+        const unsigned mh = wave_reduce<true>(lhi);
+        bool m0 = cand[0] && (unsigned)(key[0] >> 32) == mh;
+        bool m1 = cand[1] && (unsigned)(key[1] >> 32) == mh;
+        unsigned long long b0 = __ballot(m0), b1 = __ballot(m1);
+        if (__popcll(b0) + __popcll(b1) != 1) {
+            // near-tie: full 64-bit key, then the smallest swapped position
+            unsigned llo = 0u;
+#pragma unroll
+            for (int h = 0; h < RPL; ++h)
+                if (cand[h] && (unsigned)(key[h] >> 32) == mh && (unsigned)key[h] > llo)
+                    llo = (unsigned)key[h];
+            const unsigned ml = wave_reduce<true>(llo);
+            const unsigned long long vmax = ((unsigned long long)mh << 32) | ml;
+            unsigned tp = 0xffffffffu;
+#pragma unroll
+            for (int h = 0; h < RPL; ++h)
+                if (cand[h] && key[h] == vmax && (unsigned)pos[h] < tp) tp = (unsigned)pos[h];
+            const int posmin = (int)wave_reduce<false>(tp);
+            m0 = cand[0] && pos[0] == posmin;
+            m1 = cand[1] && pos[1] == posmin;
+            b0 = __ballot(m0);
+            b1 = __ballot(m1);
+        }
+        const int hp = b0 ? 0 : 1;
+        const int lp = __builtin_amdgcn_readfirstlane(__builtin_ctzll(b0 ? b0 : b1));
+        const int p = lp + 64 * hp;
+        const int posp = __builtin_amdgcn_readlane(hp ? pos[1] : pos[0], lp);
+        const double piv = hp ? readlane_d(v[1], lp) : readlane_d(v[0], lp);
+        const double xp = hp ? readlane_d(xr[1], lp) : readlane_d(xr[0], lp);
+        // multipliers of this wave's rows (published for the other waves) and
+        // the right-hand side
+        double rp = 0.0;
+        if constexpr (MODE == kGJ) rp = 1.0 / piv;
+        double l[RPL];
+#pragma unroll
+        for (int h = 0; h < RPL; ++h) {
+            const int row = G.row(h);
+            bool inc = row != p;
+            if constexpr (MODE == kLU) inc = inc && cand[h];
+            l[h] = 0.0;
+            if (inc) {
+                if constexpr (MODE == kLU) {
+                    l[h] = v[h] / piv;
+                    const double t = l[h] * xp;
+                    xr[h] = xr[h] - t;
+                } else {
+                    l[h] = v[h] * rp;
+                    xr[h] = fma(-l[h], xp, xr[h]);
+                }
+            }
+            S.buf[kk * N + row] = l[h];
+        }
+        // pivot row right of the pivot column, and the panel's own update
+        double u[TILE];
+        const int lq = p & 63;
+#pragma unroll
+        for (int c = 0; c < TILE; ++c) {
+            u[c] = 0.0;
+            if (c > kk) u[c] = hp ? readlane_d(M[TC][1][c], lq) : readlane_d(M[TC][0][c], lq);
+        }
+        if constexpr (MODE == kLU) {
+            if (G.lane == 0) {
+#pragma unroll
+                for (int c = 0; c < TILE; ++c)
+                    if (c > kk) Ul[urow_off(k) + (TILE * P + c) - k - 1] = u[c];
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < RPL; ++h)
+#pragma unroll
+            for (int c = 0; c < TILE; ++c) row_update<MODE>(M[TC][h][c], l[h], u[c]);
+        // swapped-order positions: the row at position k moves to p's place
+#pragma unroll
+        for (int h = 0; h < RPL; ++h) {
+            if (pos[h] == k) pos[h] = posp;
+            else if (G.row(h) == p) pos[h] = k;
+        }
+        if (G.lane == 0) {
+            S.prow[k] = p;
+            S.pv[k] = piv;
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < RPL; ++h) {
+        S.pos[G.row(h)] = pos[h];
+        S.x[G.row(h)] = xr[h];
+    }
+}
+
+// Applies the 16 steps of panel P, in order, to tile column slot TC.
+template <int MODE, int TC>
+__device__ __forceinline__ void apply_panel(double (&M)[TCW][RPL][TILE], Shared &S, const Geo &G,
+                                            int P, double *Ul)
+{
+    const int Jc = G.w + NWAVE * TC;
+#pragma unroll 1
+    for (int s = 0; s < TILE; ++s) {
+        const int k = TILE * P + s;
+        const int p = __builtin_amdgcn_readfirstlane(S.prow[k]);
+        double l[RPL];
+#pragma unroll
+        for (int h = 0; h < RPL; ++h) l[h] = S.buf[s * N + G.row(h)];
+        double u[TILE];
+        pivot_row<TC>(M, p, u);
+        if constexpr (MODE == kLU) {
+            if (G.lane == 0) {
+#pragma unroll
+                for (int c = 0; c < TILE; ++c) Ul[urow_off(k) + (TILE * Jc + c) - k - 1] = u[c];
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < RPL; ++h)
+#pragma unroll
+            for (int c = 0; c < TILE; ++c) row_update<MODE>(M[TC][h][c], l[h], u[c]);
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ void dense_solve(double (&M)[TCW][RPL][TILE], Shared &S, const Geo &G,
+                                            double *Ul)
+{
+#pragma unroll 1
+    for (int P = 0; P < NT; ++P) {
+        if (G.w == P % NWAVE) {
+            dispatch<0, TCW>(P / NWAVE, [&](auto TC) { factor_panel<MODE, TC.value>(M, S, G, P, Ul); });
+        }
+        __syncthreads();
+        static_for<0, TCW>([&](auto TC) {
+            if (G.w + NWAVE * CV(TC) > P) apply_panel<MODE, CV(TC)>(M, S, G, P, Ul);
+        });
+        __syncthreads();
+    }
+    if constexpr (MODE == kLU) {
+        // row-oriented back substitution in the reference's order (one lane;
+        // loads batched ahead of the dependent subtractions)
+        if (G.t == 0) {
+            for (int i = N - 1; i >= 0; --i) {
+                double xi = S.x[S.prow[i]];
+                const double *urow = Ul + urow_off(i) - (i + 1);
+                int j = i + 1;
+                for (; j + 8 <= N; j += 8) {
+                    double uv8[8], sv8[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        uv8[q] = urow[j + q];
+                        sv8[q] = S.s[j + q];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        const double t = uv8[q] * sv8[q];
+                        xi = xi - t;
+                    }
+                }
+                for (; j < N; ++j) {
+                    const double t = urow[j] * S.s[j];
+                    xi = xi - t;
+                }
+                S.s[i] = xi / S.pv[i];
+            }
+        }
+    } else {
+        if (G.t < N) S.s[G.t] = S.x[S.prow[G.t]] / S.pv[G.t];
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// BE_chem_solve (BE_chem_solve.C:79-241) for the cell in S.u (unscaled on
+// entry and exit), scaling S.sc, tolerances S.rt / S.at (atol scaled in
+// place and scaled back, as the reference does).  Returns 0 or 1.
+// ---------------------------------------------------------------------------
+template <int MODE>
+__device__ int be_solve(const NetDev &net, Shared &S, const Geo &G, double (&M)[TCW][RPL][TILE],
+                        double dt, int &nsweep, int &parity, double *Ul)
+{
+    const int t = G.t;
+    if (t < N) {
+        const double iv = 1.0 / S.sc[t];
+        S.inv[t] = iv;
+        const double uu = S.u[t] * iv;
+        S.u[t] = uu;
+        S.at[t] = S.at[t] * iv;
+        S.u0[t] = uu;
+    }
+    __syncthreads();
+    int unsolved = 0, rv = 0;
+    for (int sweep = 0; sweep < 10; ++sweep) {
+        // BE_Resid_Fun (:246-275)
+        ++nsweep;
+        if (t < N) S.u[t] = S.u[t] * S.sc[t];
+        const bool neg = t < N && S.u[t] < 0.0;
+        if (block_any(neg, S, G, parity)) {
+            rv = 1;
+            break;   // the state stays multiplied by the scale, as in the reference
+        }
+        run_rhs(net, S, G);
+        __syncthreads();
+        if (t < N) {
+            const double uu = S.u[t] * S.inv[t];
+            S.u[t] = uu;
+            double gv = S.f[t] * S.inv[t];
+            const double d = uu - S.u0[t];
+            gv = d - dt * gv;
+            S.x[t] = gv;
+            S.u[t] = uu * S.sc[t];   // BE_Resid_Jac scales the state up again (:287)
+            S.pos[t] = t;
+        }
+        __syncthreads();
+        // BE_Resid_Jac (:280-321): J = I - dt diag(inv) Jf diag(sc)
+        const double mdt = -dt;
+        static_for<0, NT>([&](auto JC) {
+            constexpr int J = CV(JC);
+            run_jac_chunk(net, S, G, J);
+            __syncthreads();
+            if (G.w == J % NWAVE) {
+                constexpr int TC = J / NWAVE;
+                double iv[RPL];
+#pragma unroll
+                for (int h = 0; h < RPL; ++h) iv[h] = S.inv[G.row(h)];
+#pragma unroll
+                for (int c = 0; c < TILE; ++c) {
+                    const int colg = TILE * J + c;
+                    const double scc = S.sc[colg];
+#pragma unroll
+                    for (int h = 0; h < RPL; ++h) {
+                        const int row = G.row(h);
+                        double val = S.buf[c * CHUNK_LD + row];
+                        val = val * iv[h];
+                        val = val * scc;
+                        val = mdt * val;
+                        if (row == colg) val = val + 1.0;
+                        M[TC][h][c] = val;
+                    }
+                }
+            }
+            __syncthreads();
+        });
+        if (t < N) S.u[t] = S.u[t] * S.inv[t];
+        dense_solve<MODE>(M, S, G, Ul);
+        // update and convergence test (:170-190)
+        bool unconv = false, isnan_ = false;
+        if (t < N) {
+            const double sv = S.s[t];
+            const double uu = S.u[t] - 1.0 * sv;
+            S.u[t] = uu;
+            const double tol = S.at[t] + S.rt[t] * fabs(uu);
+            unconv = fabs(sv) > tol;
+            isnan_ = uu != uu;
+        }
+        const unsigned long long bu = __ballot(unconv), bn = __ballot(isnan_);
+        if (G.lane == 0 && G.w < 2) {
+            S.ball[parity][2 * G.w] = bu;
+            S.ball[parity][2 * G.w + 1] = bn;
+        }
+        __syncthreads();
+        const unsigned long long u0m = S.ball[parity][0], n0m = S.ball[parity][1];
+        const unsigned long long u1m = S.ball[parity][2], n1m = S.ball[parity][3];
+        parity ^= 1;
+        const int fu = u0m ? __builtin_ctzll(u0m) : (u1m ? 64 + __builtin_ctzll(u1m) : N);
+        const int fn = n0m ? __builtin_ctzll(n0m) : (n1m ? 64 + __builtin_ctzll(n1m) : N);
+        if (fn < fu) {
+            rv = 1;
+            break;
+        }
+        unsolved = fu < N;
+        if (!unsolved) break;
+    }
+    if (rv == 0) rv = unsolved;
+    if (t < N) {
+        S.u[t] = S.u[t] * S.sc[t];
+        S.at[t] = S.at[t] * S.sc[t];
+    }
+    __syncthreads();
+    return rv;
+}
+
+__device__ __forceinline__ void cell_rates(const NetDev &net, Shared &S, const Geo &G, double T)
+{
+    const double lt = plog(T / 300.0);
+    for (int r = G.t; r < net.nrxn; r += NTHR) {
+        double e = net.beta[r] * lt;
+        e = e - net.gamma[r] / T;
+        S.k[r] = net.A[r] * pexp(e);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The substep driver of dengo_evolve_<n> (rates_and_rate_tables.C.template:
+// 174-257) for one cell per workgroup; persistent workgroups pull cells from
+// a device counter.
+// ---------------------------------------------------------------------------
+template <int MODE, bool PERSIST>
+__global__ __launch_bounds__(NTHR) void k_big(NetDev net, double dtf, double dt_in,
+                                              const double *__restrict__ Tc,
+                                              double *__restrict__ X,
+                                              const double *__restrict__ RT,
+                                              double *__restrict__ AT, long long ncells,
+                                              dengo_cell_outputs out, int max_substeps,
+                                              unsigned long long *counter)
+{
+    __shared__ Shared S;
+    __shared__ double Ul[MODE == kLU ? U_PACKED : 1];
+    const Geo G;
+    const int t = G.t;
+    double M[TCW][RPL][TILE];
+    for (int r = t; r < net.nrxn; r += NTHR) {
+        const int4 q = net.rxn[r];
+        S.ab[r] = (unsigned short)(q.x | (q.y << 8));
+    }
+    int parity = 0;
+    for (long long iter = 0;; ++iter) {
+        long long cell;
+        if constexpr (PERSIST) {
+            if (t == 0) S.cell = (long long)atomicAdd(counter, 1ULL);
+            __syncthreads();
+            cell = S.cell;
+        } else {
+            if (iter > 0) break;
+            cell = blockIdx.x;
+        }
+        if (cell >= ncells) break;
+        const double T = Tc[cell];
+        if (t < N) {
+            const double v = X[cell * N + t];
+            S.u[t] = v;
+            S.sc[t] = v;
+            S.prev[t] = v;
+            S.rt[t] = RT[cell * N + t];
+            S.at[t] = AT[cell * N + t];
+        }
+        cell_rates(net, S, G, T);
+        __syncthreads();
+        double dt = dt_in < 0.0 ? dtf / 1e5 : dt_in;
+        double ttot = 0.0;
+        int siter = 0, nrej = 0, nsweep = 0, status = 0;
+        while (ttot < dtf) {
+            int rv = be_solve<MODE>(net, S, G, M, dt, nsweep, parity, Ul);
+            if (rv == 0) {
+                const bool neg = t < N && S.u[t] < 0.0;
+                if (block_any(neg, S, G, parity)) rv = 1;
+            }
+            if (rv == 0) {
+                if (siter == max_substeps) {
+                    status = 2;
+                    break;
+                }
+                ++siter;
+                ttot += dt;
+                dt = (dt * 1.1 < dtf - ttot) ? dt * 1.1 : dtf - ttot;
+                if (t < N) {
+                    const double v = S.u[t];
+                    S.prev[t] = v;
+                    if (v < 1e-25) S.u[t] = 1e-25;
+                }
+            } else {
+                ++nrej;
+                dt /= 2.0;
+                if (t < N) S.u[t] = S.prev[t];
+                if (dt / dtf < 1e-15) {
+                    status = 1;
+                    break;
+                }
+            }
+            __syncthreads();
+        }
+        if (t < N) {
+            X[cell * N + t] = S.u[t];
+            AT[cell * N + t] = S.at[t];
+        }
+        if (t == 0) {
+            if (out.status) out.status[cell] = status;
+            if (out.nsub) out.nsub[cell] = siter;
+            if (out.nrej) out.nrej[cell] = nrej;
+            if (out.nsweep) out.nsweep[cell] = nsweep;
+            if (out.dt) out.dt[cell] = dt;
+            if (out.ttot) out.ttot[cell] = ttot;
+            if (out.T) out.T[cell] = T;
+        }
+        __syncthreads();
+    }
+}
+
+// Batched RHS and Jacobian (column-major per cell, J[cell][col][row]) of
+// number-density states; neg[cell] = 1 when a density is negative (the
+// reference's RHS then returns 1 without evaluating).
+__global__ __launch_bounds__(NTHR) void k_big_eval(NetDev net, const double *__restrict__ Tc,
+                                                   const double *__restrict__ U, long long ncells,
+                                                   double *__restrict__ F, double *__restrict__ Jout,
+                                                   int *__restrict__ neg)
+{
+    __shared__ Shared S;
+    const Geo G;
+    const int t = G.t;
+    const long long cell = blockIdx.x;
+    if (cell >= ncells) return;
+    for (int r = t; r < net.nrxn; r += NTHR) {
+        const int4 q = net.rxn[r];
+        S.ab[r] = (unsigned short)(q.x | (q.y << 8));
+    }
+    if (t < N) S.u[t] = U[cell * N + t];
+    cell_rates(net, S, G, Tc[cell]);
+    int parity = 0;
+    const bool ng = t < N && S.u[t] < 0.0;
+    const bool any = block_any(ng, S, G, parity);
+    if (t == 0) neg[cell] = any ? 1 : 0;
+    if (any) return;
+    run_rhs(net, S, G);
+    __syncthreads();
+    if (t < N) F[cell * N + t] = S.f[t];
+    __syncthreads();
+    if (!Jout) return;
+    for (int J = 0; J < NT; ++J) {
+        run_jac_chunk(net, S, G, J);
+        __syncthreads();
+        for (int e = t; e < TILE * N; e += NTHR) {
+            const int lc = e / N, row = e % N;
+            Jout[(size_t)cell * N * N + (size_t)(TILE * J + lc) * N + row] = S.buf[lc * CHUNK_LD + row];
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace big
+}  // namespace dengo

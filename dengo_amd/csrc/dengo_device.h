@@ -68,6 +68,35 @@ DENGO_HD double plog(double x)
     return hi + r;
 }
 
+// exp(x) from basic operations only, so CPU and GPU builds agree bit for bit:
+// x = n ln2 + r with n = floor(x / ln2 + 1/2) and the two-part ln2 above
+// (Cody & Waite), exp(r) by its Taylor series to degree 13 in Horner form
+// (|r| <= 0.347, truncation < 5e-18 relative), then exp(x) = 2^n exp(r).
+DENGO_HD double pexp(double x)
+{
+    if (x != x) return x;
+    if (x > 709.782712893384) return __builtin_huge_val();
+    if (x < -745.2) return 0.0;
+    const double n = floor(x * 1.4426950408889634 + 0.5);
+    double r = x - n * 0.6931471803691238;
+    r = r - n * 1.9082149292705877e-10;
+    double p = 1.6059043836821613e-10;
+    p = 2.08767569878681e-09 + r * p;
+    p = 2.505210838544172e-08 + r * p;
+    p = 2.755731922398589e-07 + r * p;
+    p = 2.7557319223985893e-06 + r * p;
+    p = 2.48015873015873e-05 + r * p;
+    p = 0.0001984126984126984 + r * p;
+    p = 0.001388888888888889 + r * p;
+    p = 0.008333333333333333 + r * p;
+    p = 0.041666666666666664 + r * p;
+    p = 0.16666666666666666 + r * p;
+    p = 0.5 + r * p;
+    p = 1.0 + r * p;
+    p = 1.0 + r * p;
+    return ldexp(p, (int)n);
+}
+
 // Rate tables on the device: for bin b in [0, nbins) and table t,
 // pairs[b * ntab + t] = (table_t[b], table_t[b + 1]), so one cell reads its
 // bin as one contiguous run of 16-byte loads.

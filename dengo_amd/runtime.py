@@ -82,6 +82,22 @@ SIGNATURES = {
 }
 for _n in NETWORKS:
     SIGNATURES.update(_network_signatures(_n))
+SIGNATURES.update({
+    "synthetic_setup_data": (ctypes.c_void_p, [ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, _dp]),
+    "synthetic_free_data": (None, [ctypes.c_void_p]),
+    "dengo_evolve_synthetic": (ctypes.c_double, [ctypes.c_double, _dp, _dp, _dp, _dp, _dp,
+                                                 ctypes.c_longlong, ctypes.c_void_p]),
+    "dengo_synthetic_evolve_device": (
+        ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Options), ctypes.c_double, ctypes.c_double,
+                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                       ctypes.c_longlong, ctypes.POINTER(CellOutputs), _dp, _dp,
+                       ctypes.c_void_p]),
+    "dengo_synthetic_evolve_host": (
+        ctypes.c_double, [ctypes.c_void_p, ctypes.POINTER(Options), ctypes.c_double, _dp, _dp,
+                          _dp, _dp, _dp, ctypes.c_longlong, _ip, _ip, _ip, _ip, _dp]),
+    "dengo_synthetic_eval": (ctypes.c_longlong, [ctypes.c_void_p, _dp, _dp, ctypes.c_longlong,
+                                                 _dp, _dp]),
+})
 
 
 def load(path=LIB_PATH):
@@ -263,3 +279,100 @@ class OxygenSolver(NetworkSolver):
             from .networks import oxygen_tables
             _, tables = oxygen_tables(T_bounds=T_bounds)
         NetworkSolver.__init__(self, "oxygen", tables, T_bounds)
+
+
+class SyntheticSolver(object):
+    """The dense-Jacobian solver for a synthetic network (128 species,
+    reactions a + b -> c + d, Arrhenius rates, isothermal cells)."""
+    N = 128
+
+    def __init__(self, net):
+        lib = load()
+        self.net = net
+        rxn = np.ascontiguousarray(net.rxn, dtype=np.int32).ravel()
+        A = np.ascontiguousarray(net.A, dtype=np.float64)
+        beta = np.ascontiguousarray(net.beta, dtype=np.float64)
+        gamma = np.ascontiguousarray(net.gamma, dtype=np.float64)
+        self.data = lib.synthetic_setup_data(net.n, net.nrxn, _p(rxn, _ip), _p(A), _p(beta),
+                                             _p(gamma))
+        if not self.data:
+            raise RuntimeError("synthetic_setup_data failed: %s" % last_error())
+
+    def close(self):
+        if getattr(self, "data", None):
+            load().synthetic_free_data(self.data)
+            self.data = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def evolve(self, x, T, rtol, atol, dtf, dt=-1.0, **opt):
+        """Integration of number densities x[n, 128] at temperatures T[n] on
+        host copies; returns a dict like NetworkSolver.evolve."""
+        out = np.array(x, dtype=np.float64, copy=True, order="C")
+        at = np.array(atol, dtype=np.float64, copy=True, order="C")
+        rt = np.array(rtol, dtype=np.float64, copy=True, order="C")
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        if out.ndim != 2 or out.shape[1] != self.N or T.shape != (out.shape[0],):
+            raise ValueError("expected x of shape [n, %d] and T of shape [n]" % self.N)
+        n = out.shape[0]
+        res = dict(status=np.zeros(n, np.int32), nsub=np.zeros(n, np.int32),
+                   nrej=np.zeros(n, np.int32), nsweep=np.zeros(n, np.int32),
+                   dt_cell=np.zeros(n))
+        o = options(**opt)
+        o.block = 0
+        dtc = ctypes.c_double(dt)
+        ttot = load().dengo_synthetic_evolve_host(
+            self.data, ctypes.byref(o), dtf, ctypes.byref(dtc), _p(T), _p(out), _p(rt), _p(at), n,
+            _p(res["status"], _ip), _p(res["nsub"], _ip), _p(res["nrej"], _ip),
+            _p(res["nsweep"], _ip), _p(res["dt_cell"]))
+        if ttot < 0:
+            raise RuntimeError("evolve failed: %s" % last_error())
+        res.update(state=out, atol=at, ttot=ttot, dt=dtc.value,
+                   kernel_ms=load().dengo_last_kernel_ms())
+        return res
+
+    def evolve_device(self, x, T, rtol, atol, dtf, dt=-1.0, outputs=None, stream=None,
+                      sync_min=True, **opt):
+        """In-place integration of device-resident float64 torch tensors x,
+        rtol, atol [n, 128] at T [n]; see NetworkSolver.evolve_device."""
+        for t in (x, rtol, atol):
+            if not (t.is_cuda and t.dtype.is_floating_point and t.element_size() == 8
+                    and t.is_contiguous() and t.dim() == 2 and t.shape[1] == self.N):
+                raise ValueError("expected contiguous float64 cuda tensors [n, %d]" % self.N)
+        n = x.shape[0]
+        if not (T.is_cuda and T.element_size() == 8 and T.is_contiguous() and T.numel() >= n):
+            raise ValueError("expected a contiguous float64 cuda tensor T[n]")
+        co = CellOutputs()
+        for k, v in (outputs or {}).items():
+            if v.numel() < n:
+                raise ValueError("output %s too small" % k)
+            setattr(co, k, v.data_ptr())
+        o = options(**opt)
+        o.block = 0
+        tmin = ctypes.c_double(0.0)
+        dmin = ctypes.c_double(0.0)
+        s = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+        rc = load().dengo_synthetic_evolve_device(
+            self.data, ctypes.byref(o), dtf, dt, ctypes.c_void_p(T.data_ptr()),
+            ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(rtol.data_ptr()),
+            ctypes.c_void_p(atol.data_ptr()), n, ctypes.byref(co),
+            ctypes.byref(tmin) if sync_min else None, ctypes.byref(dmin) if sync_min else None, s)
+        _check(rc, "dengo_synthetic_evolve_device")
+        return (tmin.value, dmin.value) if sync_min else None
+
+    def eval(self, x, T, jacobian=True):
+        """RHS f[n,128], Jacobian J[n,128,128] (J[c, col, row]) and the count
+        of cells with a negative density."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        n = x.shape[0]
+        f = np.zeros((n, self.N))
+        J = np.zeros((n, self.N, self.N)) if jacobian else None
+        r = load().dengo_synthetic_eval(self.data, _p(T), _p(x), n, _p(f), _p(J))
+        if r < 0:
+            raise RuntimeError("eval failed: %s" % last_error())
+        return f, J, int(r)

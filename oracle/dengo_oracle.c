@@ -257,6 +257,26 @@ double orc_plog(double x)
     return hi + r;
 }
 
+/* exp(x) from basic operations (Cody & Waite reduction with the two-part ln2
+ * above, degree-13 Taylor series in Horner form, scaling by 2^n), so the
+ * oracle and the GPU evaluate it bit for bit alike. */
+double orc_pexp(double x)
+{
+    if (x != x) return x;
+    if (x > 709.782712893384) return HUGE_VAL;
+    if (x < -745.2) return 0.0;
+    static const double c[14] = {1.0, 1.0, 0.5, 0.16666666666666666, 0.041666666666666664,
+        0.008333333333333333, 0.001388888888888889, 0.0001984126984126984,
+        2.48015873015873e-05, 2.7557319223985893e-06, 2.755731922398589e-07,
+        2.505210838544172e-08, 2.08767569878681e-09, 1.6059043836821613e-10};
+    double n = floor(x * 1.4426950408889634 + 0.5);
+    double r = x - n * 0.6931471803691238;
+    r = r - n * 1.9082149292705877e-10;
+    double p = c[13];
+    for (int k = 12; k >= 0; --k) p = c[k] + r * p;
+    return ldexp(p, (int)n);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Temperature, interpolation, RHS, Jacobian                                 */
 /* ------------------------------------------------------------------------ */
@@ -670,8 +690,9 @@ int orc_BE_chem_solve(orc_rhs_f f, orc_jac_f J, double *u, double dt, double *rt
                       double *atol, int nstrip, int n, double *sc, void *sdata,
                       double *u0, double *s, double *gu, double *Ju)
 {
-    double inv[64];
-    int unsolved = 0;
+    double inv_small[64];
+    double *inv = n <= 64 ? inv_small : (double *)malloc(sizeof(double) * n);
+    int unsolved = 0, rv = 0;
     (void)nstrip;
     for (int i = 0; i < n; ++i) inv[i] = 1.0 / sc[i];
     for (int i = 0; i < n; ++i) u[i] *= inv[i];
@@ -683,9 +704,8 @@ int orc_BE_chem_solve(orc_rhs_f f, orc_jac_f J, double *u, double dt, double *rt
     for (int sweep = 0; sweep < 10; ++sweep) {
         if (be_resid(f, u, u0, gu, dt, n, sc, inv, sdata) != 0 ||
             be_jac(J, u, Ju, dt, n, sc, inv, sdata) != 0) {
-            for (int i = 0; i < n; ++i) u[i] *= sc[i];
-            for (int i = 0; i < n; ++i) atol[i] *= sc[i];
-            return 1;
+            rv = 1;
+            goto done;
         }
         unsolved = 0;
         orc_gauss_elim(Ju, s, gu, n);
@@ -696,16 +716,18 @@ int orc_BE_chem_solve(orc_rhs_f f, orc_jac_f J, double *u, double dt, double *rt
                 break;
             }
             if (u[i] != u[i]) {
-                for (int k = 0; k < n; ++k) u[k] *= sc[k];
-                for (int k = 0; k < n; ++k) atol[k] *= sc[k];
-                return 1;
+                rv = 1;
+                goto done;
             }
         }
         if (!unsolved) break;
     }
+    rv = unsolved ? 1 : 0;
+done:
     for (int i = 0; i < n; ++i) u[i] *= sc[i];
     for (int i = 0; i < n; ++i) atol[i] *= sc[i];
-    return unsolved ? 1 : 0;
+    if (inv != inv_small) free(inv);
+    return rv;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -896,5 +918,189 @@ int orc_prim_eval(const double *u, long long dims, const double *tables, int nT,
         if (T) T[c] = d.T;
         if (rates) memcpy(rates + c * ORC_PRIM_NTAB, d.rs, sizeof(double) * ORC_PRIM_NTAB);
     }
+    return nneg;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Large synthetic networks: species X_0..X_{n-1}, two-body reactions        */
+/* a + b -> c + d with Arrhenius k = A (T/300)^beta exp(-gamma/T) at the     */
+/* cell's fixed temperature; no energy equation (isothermal cells).  The    */
+/* BE/Newton/substep semantics are the reference's (BE_chem_solve.C:79-382, */
+/* rates_and_rate_tables.C.template:174-257), number densities throughout.  */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    const orc_big_net *net;
+    const double *k;
+    long long rhs_calls;
+} big_data;
+
+static int big_rhs(double *u, double *f, int nstrip, int nchem, void *sdata)
+{
+    big_data *d = (big_data *)sdata;
+    const orc_big_net *net = d->net;
+    (void)nstrip;
+    d->rhs_calls++;
+    for (int s = 0; s < nchem; ++s)
+        if (u[s] < 0.0) return 1;
+    for (int s = 0; s < nchem; ++s) f[s] = 0.0;
+    for (int r = 0; r < net->nrxn; ++r) {
+        const int *sp = net->rxn + 4 * r;   /* a, b, c, d */
+        double flux = d->k[r];
+        flux = flux * u[sp[0]];
+        flux = flux * u[sp[1]];
+        /* net change of each distinct species, ascending species index */
+        int idx[4], cnt = 0;
+        for (int q = 0; q < 4; ++q) {
+            int seen = 0;
+            for (int t = 0; t < cnt; ++t) seen |= idx[t] == sp[q];
+            if (!seen) idx[cnt++] = sp[q];
+        }
+        for (int a = 1; a < cnt; ++a)
+            for (int b = a; b > 0 && idx[b - 1] > idx[b]; --b) {
+                int t = idx[b]; idx[b] = idx[b - 1]; idx[b - 1] = t;
+            }
+        for (int t = 0; t < cnt; ++t) {
+            int s = idx[t], nu = 0;
+            nu += (sp[2] == s) + (sp[3] == s) - (sp[0] == s) - (sp[1] == s);
+            if (nu != 0) f[s] += (double)nu * flux;
+        }
+    }
+    return 0;
+}
+
+static int big_jac(double *u, double *J, int nstrip, int nchem, void *sdata)
+{
+    big_data *d = (big_data *)sdata;
+    const orc_big_net *net = d->net;
+    const int n = nchem;
+    (void)nstrip;
+    for (int s = 0; s < n; ++s)
+        if (u[s] < 0.0) return 1;
+    for (long long q = 0; q < (long long)n * n; ++q) J[q] = 0.0;
+    for (int r = 0; r < net->nrxn; ++r) {
+        const int *sp = net->rxn + 4 * r;
+        for (int j = 0; j < 2; ++j) {
+            double part = d->k[r];
+            part = part * u[sp[1 - j]];
+            int col = sp[j];
+            int idx[4], cnt = 0;
+            for (int q = 0; q < 4; ++q) {
+                int seen = 0;
+                for (int t = 0; t < cnt; ++t) seen |= idx[t] == sp[q];
+                if (!seen) idx[cnt++] = sp[q];
+            }
+            for (int t = 0; t < cnt; ++t) {
+                int s = idx[t];
+                int nu = (sp[2] == s) + (sp[3] == s) - (sp[0] == s) - (sp[1] == s);
+                if (nu != 0) J[(long long)col * n + s] += (double)nu * part;
+            }
+        }
+    }
+    return 0;
+}
+
+/* k = A (T/300)^beta exp(-gamma/T), evaluated as A exp(beta ln(T/300) -
+ * gamma/T) with the portable log and exp. */
+void orc_big_rates(const orc_big_net *net, double T, double *k)
+{
+    const double lt = orc_plog(T / 300.0);
+    for (int r = 0; r < net->nrxn; ++r) {
+        double e = net->beta[r] * lt;
+        e = e - net->gamma[r] / T;
+        k[r] = net->A[r] * orc_pexp(e);
+    }
+}
+
+double orc_big_evolve(const orc_big_net *net, double dtf, double *dt, const double *Tcell,
+                      double *input, double *rtol, double *atol, long long dims, int *status,
+                      int *nsub, int *nrej, int *nsweep, double *dt_out, int nthreads)
+{
+    const int n = net->n;
+    const double dt_in = *dt;
+    double tmin = 1e300, dtmin = dt_in;
+    long long imin = -1;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+#endif
+    for (long long c = 0; c < dims; ++c) {
+        double *k = (double *)malloc(sizeof(double) * net->nrxn);
+        double *scale = (double *)malloc(sizeof(double) * n * 5);
+        double *prev = scale + n, *u0 = scale + 2 * n, *s = scale + 3 * n, *gu = scale + 4 * n;
+        double *Ju = (double *)malloc(sizeof(double) * n * n);
+        double *in = input + c * n, *rt = rtol + c * n, *at = atol + c * n;
+        big_data d = {net, k, 0};
+        orc_big_rates(net, Tcell[c], k);
+        double dtc = dt_in, ttot = 0.0;
+        int siter = 0, st = 0, rej = 0;
+        if (dtc < 0) dtc = dtf / 1e5;
+        for (int q = 0; q < n; ++q) scale[q] = in[q];
+        for (int q = 0; q < n; ++q) prev[q] = in[q];
+        while (ttot < dtf) {
+            int rv = orc_BE_chem_solve(big_rhs, big_jac, in, dtc, rt, at, 1, n, scale, &d, u0, s,
+                                       gu, Ju);
+            for (int q = 0; q < n; ++q)
+                if (in[q] < 0) { rv = 1; break; }
+            if (rv == 0) {
+                if (siter == g_max_substeps) { st = 2; break; }
+                siter++;
+                ttot += dtc;
+                dtc = (dtc * 1.1 < dtf - ttot) ? dtc * 1.1 : dtf - ttot;
+                for (int q = 0; q < n; ++q) prev[q] = in[q];
+                for (int q = 0; q < n; ++q)
+                    if (in[q] < 1e-25) in[q] = 1e-25;
+            } else {
+                rej++;
+                dtc /= 2.0;
+                for (int q = 0; q < n; ++q) in[q] = prev[q];
+                if (dtc / dtf < 1e-15) { st = 1; break; }
+            }
+        }
+        if (status) status[c] = st;
+        if (nsub) nsub[c] = siter;
+        if (nrej) nrej[c] = rej;
+        if (nsweep) nsweep[c] = (int)d.rhs_calls;
+        if (dt_out) dt_out[c] = dtc;
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+        {
+            if (ttot < tmin || (ttot == tmin && c < imin)) {
+                tmin = ttot;
+                dtmin = dtc;
+                imin = c;
+            }
+        }
+        free(k);
+        free(scale);
+        free(Ju);
+    }
+    (void)nthreads;
+    if (dims > 0) *dt = dtmin;
+    return dims > 0 ? tmin : 0.0;
+}
+
+/* RHS f[dims][n] and Jacobian J[dims][n][n] (column-major per cell) of
+ * number-density states; cells with a negative density are skipped and
+ * counted, as the RHS then returns 1 without evaluating. */
+long long orc_big_eval(const orc_big_net *net, const double *Tcell, const double *u,
+                       long long dims, double *f, double *J)
+{
+    const int n = net->n;
+    long long nneg = 0;
+    double *k = (double *)malloc(sizeof(double) * net->nrxn);
+    double *uc = (double *)malloc(sizeof(double) * n);
+    for (long long c = 0; c < dims; ++c) {
+        big_data d = {net, k, 0};
+        orc_big_rates(net, Tcell[c], k);
+        memcpy(uc, u + c * n, sizeof(double) * n);
+        if (big_rhs(uc, f + c * n, 1, n, &d) != 0) {
+            nneg++;
+            continue;
+        }
+        if (J) big_jac(uc, J + c * (long long)n * n, 1, n, &d);
+    }
+    free(k);
+    free(uc);
     return nneg;
 }

@@ -117,6 +117,25 @@ int orc_prim_eval(const double *u, long long dims, const double *tables, int nT,
                   double T0, double T1, double z, double *f, double *J,
                   double *T, double *rates, int flags);
 
+double orc_pexp(double x);
+
+/* Large synthetic network: rxn[4 r .. 4 r + 3] = (a, b, c, d) for
+ * a + b -> c + d, Arrhenius k = A (T/300)^beta exp(-gamma/T). */
+typedef struct orc_big_net {
+    int n, nrxn;
+    const int *rxn;
+    const double *A, *beta, *gamma;
+} orc_big_net;
+void orc_big_rates(const orc_big_net *net, double T, double *k);
+/* One isothermal BE integration per cell (number densities, no energy
+ * equation); per-cell outputs may be NULL. */
+double orc_big_evolve(const orc_big_net *net, double dtf, double *dt, const double *Tcell,
+                      double *input, double *rtol, double *atol, long long dims, int *status,
+                      int *nsub, int *nrej, int *nsweep, double *dt_out, int nthreads);
+
+long long orc_big_eval(const orc_big_net *net, const double *Tcell, const double *u,
+                       long long dims, double *f, double *J);
+
 /* Mass-action network counterparts of orc_prim_evolve / orc_prim_eval. */
 double orc_ma_evolve(const orc_ma_net *net, double dtf, double *dt, double z, double *input,
                      double *rtol, double *atol, long long dims, const double *tables,

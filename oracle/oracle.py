@@ -234,3 +234,73 @@ def ma_evolve(net, inp, rtol, atol, tables, dtf, T_bounds, dt=-1.0, z=-1.0,
         _ptr(T_out), None, nthreads, flags)
     return dict(state=out, atol=atol_o, ttot=ttot, dt=dtc.value, status=status, nsub=nsub,
                 nrej=nrej, nsweep=nsweep, dt_cell=dt_out, T=T_out)
+
+
+class BigNet(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("nrxn", ctypes.c_int), ("rxn", _ip),
+                ("A", _dp), ("beta", _dp), ("gamma", _dp)]
+
+
+def big_net(net):
+    """orc_big_net view of a SyntheticNetwork (keeps its arrays alive)."""
+    rxn = np.ascontiguousarray(net.rxn, dtype=np.int32).ravel()
+    A = np.ascontiguousarray(net.A, dtype=np.float64)
+    beta = np.ascontiguousarray(net.beta, dtype=np.float64)
+    gamma = np.ascontiguousarray(net.gamma, dtype=np.float64)
+    s = BigNet(net.n, net.nrxn, _ptr(rxn, _ip), _ptr(A), _ptr(beta), _ptr(gamma))
+    s._keep = (rxn, A, beta, gamma)
+    return s
+
+
+def big_rates(net, T):
+    lib = load()
+    lib.orc_big_rates.restype = None
+    lib.orc_big_rates.argtypes = [ctypes.POINTER(BigNet), ctypes.c_double, _dp]
+    bn = big_net(net)
+    out = np.zeros((len(T), net.nrxn))
+    for c, t in enumerate(T):
+        lib.orc_big_rates(ctypes.byref(bn), float(t), _ptr(out[c]))
+    return out
+
+
+def big_evolve(net, x, T, rtol, atol, dtf, dt=-1.0, nthreads=0, max_substeps=None):
+    """Isothermal BE integration of the synthetic network, per cell.  Returns
+    (ttot, dt, outputs dict) and updates x (number densities) in place."""
+    lib = load()
+    lib.orc_big_evolve.restype = ctypes.c_double
+    lib.orc_big_evolve.argtypes = [ctypes.POINTER(BigNet), ctypes.c_double, _dp, _dp, _dp, _dp,
+                                   _dp, ctypes.c_longlong, _ip, _ip, _ip, _ip, _dp, ctypes.c_int]
+    assert x.dtype == np.float64 and x.flags.c_contiguous
+    ncells = x.shape[0]
+    bn = big_net(net)
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    at = np.ascontiguousarray(atol, dtype=np.float64)
+    rt = np.ascontiguousarray(rtol, dtype=np.float64)
+    out = {k: np.zeros(ncells, dtype=np.int32) for k in ("status", "nsub", "nrej", "nsweep")}
+    out["dt"] = np.zeros(ncells)
+    dtv = ctypes.c_double(dt)
+    if max_substeps is not None:
+        lib.orc_set_max_substeps(int(max_substeps))
+    try:
+        ttot = lib.orc_big_evolve(ctypes.byref(bn), dtf, ctypes.byref(dtv), _ptr(T), _ptr(x),
+                                  _ptr(rt), _ptr(at), ncells, _ptr(out["status"], _ip),
+                                  _ptr(out["nsub"], _ip), _ptr(out["nrej"], _ip),
+                                  _ptr(out["nsweep"], _ip), _ptr(out["dt"]), nthreads)
+    finally:
+        if max_substeps is not None:
+            lib.orc_set_max_substeps(50000)
+    return ttot, dtv.value, out
+
+
+def big_eval(net, T, u):
+    """(f, J, nneg) of the synthetic network at number densities u[cells, n]."""
+    lib = load()
+    lib.orc_big_eval.restype = ctypes.c_longlong
+    lib.orc_big_eval.argtypes = [ctypes.POINTER(BigNet), _dp, _dp, ctypes.c_longlong, _dp, _dp]
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    f = np.zeros_like(u)
+    J = np.zeros((u.shape[0], net.n, net.n))
+    bn = big_net(net)
+    nneg = lib.orc_big_eval(ctypes.byref(bn), _ptr(T), _ptr(u), u.shape[0], _ptr(f), _ptr(J))
+    return f, J, nneg
