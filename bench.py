@@ -47,6 +47,46 @@ def log(msg):
     sys.stderr.flush()
 
 
+def serving_leg(args, prim):
+    """Poisson arrivals of patches (edge 8-64 cells, primordial:oxygen:synthetic
+    = 6:3:1) through the patch server, mixed precision; returns the latency
+    and throughput summary."""
+    from dengo_amd import runtime, serving
+    from dengo_amd.synthetic import SyntheticNetwork
+    oxy = runtime.OxygenSolver()
+    net = SyntheticNetwork(seed=7)
+    syn = runtime.SyntheticSolver(net)
+    trace = serving.poisson_trace(args.serve_rate, args.serve_seconds, seed=args.seed)
+    log("serving: %d patches, %d cells; building inputs" % (len(trace),
+                                                            sum(t[2] for t in trace)))
+    patches, keep = [], []
+    for (_, k, n, s) in trace:
+        p, kp = serving.make_patch(k, n, s, synthetic_net=net)
+        patches.append(p)
+        keep.append(kp)
+    srv = runtime.PatchServer(prim, oxy, syn, precision=runtime.MIXED, slots=args.serve_slots,
+                              grid_share=args.serve_share)
+    try:
+        recs, t0 = serving.replay(srv, trace, patches, 1e12, timeout_s=900.0)
+        stats = srv.stats()
+    finally:
+        srv.close()
+    out = serving.summarize(trace, recs, t0)
+    out.update({"rate_per_s": args.serve_rate, "window_s": args.serve_seconds,
+                "slots": args.serve_slots, "grid_share": args.serve_share or args.serve_slots,
+                "precision": "mixed (primordial, oxygen); fp64-fast (synthetic dense)",
+                "arena_peak_bytes": stats["arena_peak"], "arena_bytes": stats["arena_bytes"],
+                "latency": "submit (host arrays in pinned memory) to copy-back complete"})
+    log("serving: p50 %.1f ms, p99 %.1f ms, %.0f cell-solves/s" % (
+        out["latency_ms_p50"], out["latency_ms_p99"], out["cell_solves_per_s"] or 0))
+    for kp in keep:
+        for a in kp:
+            a.free()
+    oxy.close()
+    syn.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -62,6 +102,13 @@ def main():
     ap.add_argument("--no-percell", action="store_true", help="skip the plain per-cell kernel")
     ap.add_argument("--cpu-cells", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=2.0,
+                    help="patch server: Poisson arrival rate (patches/s)")
+    ap.add_argument("--serve-seconds", type=float, default=15.0,
+                    help="patch server: arrival window (s); 0 skips the serving leg")
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-share", type=int, default=0,
+                    help="patch server: grid share (0: the number of slots)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -192,6 +239,10 @@ def main():
             "traffic": None, "flops_per_launch": flops, "F_sweep": F_SWEEP,
             "sweeps_per_launch": sweeps}
 
+    serve = None
+    if args.serve_seconds > 0:
+        serve = serving_leg(args, solver)
+
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -224,6 +275,7 @@ def main():
         "other_variants": extra,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "serving": serve,
     }
     print(json.dumps(out))
 

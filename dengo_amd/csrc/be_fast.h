@@ -315,12 +315,17 @@ DENGO_D void begin_be_fast(Cell<Net> &c)
     c.sweep = 0;
 }
 
-// Iterative-refinement steps of the FP32 solve (each: one FP64 residual of the
-// block and one FP32 triangular solve).
+// Iterative refinement of the FP32 solve: at most kRefineSteps steps (each one
+// FP64 residual of the block and one FP32 triangular solve), stopped once a
+// correction is below kRefineTol of the Newton test's tolerance in every
+// component; a solve that has not got there is redone by the FP64 LU of the
+// same block, so an FP32 factor that cannot resolve the step (condition
+// number near 1/eps_fp32) never decides the Newton iteration.
 #ifndef DENGO_REFINE_STEPS
-#define DENGO_REFINE_STEPS 1
+#define DENGO_REFINE_STEPS 3
 #endif
 constexpr int kRefineSteps = DENGO_REFINE_STEPS;
+constexpr double kRefineTol = 1e-3;
 
 template <class Net, class LUT>
 DENGO_D int sweep_fast(Cell<Net> &c, const KParams<Net> &p)
@@ -392,8 +397,12 @@ DENGO_D int sweep_fast(Cell<Net> &c, const KParams<Net> &p)
         block_lu_apply<M, FT::plan>(A32, rp, piv, y32);
 #pragma unroll
         for (int i = 0; i < M; ++i) y[i] = e[i] * (double)y32[i];
+        double tol[M];
 #pragma unroll
-        for (int it = 0; it < kRefineSteps; ++it) {
+        for (int i = 0; i < M; ++i)
+            tol[i] = kRefineTol * (c.atol[i] + c.rtol[i] * fabs(c.u[i] * c.inv[i]));
+        bool refined = false;
+        for (int it = 0; it < kRefineSteps && !refined; ++it) {
             static_for<0, M>([&](auto ii) {
                 constexpr int i = CV(ii);
                 double r = g[i];
@@ -404,8 +413,21 @@ DENGO_D int sweep_fast(Cell<Net> &c, const KParams<Net> &p)
                 y32[i] = (float)(r * d[i]);
             });
             block_lu_apply<M, FT::plan>(A32, rp, piv, y32);
+            bool small = true;
 #pragma unroll
-            for (int i = 0; i < M; ++i) y[i] = y[i] + e[i] * (double)y32[i];
+            for (int i = 0; i < M; ++i) {
+                const double dy = e[i] * (double)y32[i];
+                y[i] = y[i] + dy;
+                small = small && fabs(dy) <= tol[i];
+            }
+            refined = small;
+        }
+        if (!refined) {
+            double rp64[M];
+#pragma unroll
+            for (int i = 0; i < M; ++i) y[i] = g[i];
+            block_lu_factor<M, FT::plan>(A, rp64, piv);
+            block_lu_apply<M, FT::plan>(A, rp64, piv, y);
         }
     }
     double yge = g[GE];

@@ -30,3 +30,6 @@ using OxyHost = dengo::Host<dengo::MANet<dengo::net_oxygen::Desc>, OxyTraits>;
 }  // namespace
 
 DENGO_DEFINE_C_ABI(oxygen, OxyHost)
+
+dengo::NetHooks dengo::hooks_oxygen() { return OxyHost::hooks(); }
+

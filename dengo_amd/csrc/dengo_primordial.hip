@@ -29,3 +29,6 @@ using PrimHost = dengo::Host<dengo::PrimNet, PrimTraits>;
 }  // namespace
 
 DENGO_DEFINE_C_ABI(primordial, PrimHost)
+
+dengo::NetHooks dengo::hooks_primordial() { return PrimHost::hooks(); }
+

@@ -187,6 +187,34 @@ const char *last_error_text();
                                       __FILE__, __LINE__);                               \
     } while (0)
 
+// One integration enqueued by the patch server: a network-independent view of
+// the arguments of <n>_evolve_device with the queue counter made explicit.
+struct LaunchArgs {
+    dengo_options opt;
+    double dtf, dt0, z;
+    const double *T;              // per-cell temperatures (isothermal dense network)
+    double *x;
+    const double *rtol;
+    double *atol;
+    long long n;
+    dengo_cell_outputs out;
+    hipStream_t stream;
+    unsigned long long *counter;  // persistent schedule's work counter
+};
+
+// What the patch server needs of a network: prepare() makes the tables
+// resident on the current device and reports the persistent grid that fills
+// it for the given options; launch() enqueues one integration.
+struct NetHooks {
+    int nspecies;
+    int isothermal;               // 1: cells carry T[n] instead of an energy unknown
+    int (*prepare)(void *data, const dengo_options *opt, int *full_grid);
+    int (*launch)(void *data, const LaunchArgs &a);
+};
+NetHooks hooks_primordial();
+NetHooks hooks_oxygen();
+NetHooks hooks_synthetic();
+
 struct DevState {
     int device = -1;
     double2 *pairs = nullptr;
@@ -412,9 +440,28 @@ struct Host {
         *kc = k_chem<Net, false, MODE>;
     }
 
+    // Workgroups of a persistent launch that fill the GPU at full occupancy.
+    static int full_grid(DevState *s, const dengo_options &o, int *grid)
+    {
+        void (*kp)(KParams<Net>) = nullptr;
+        void (*kc)(KParams<Net>) = nullptr;
+        if (o.precision == DENGO_FP64) kernels<kFaithful>(&kp, &kc);
+        else if (o.precision == DENGO_FP64_FAST) kernels<kFast64>(&kp, &kc);
+        else kernels<kMixed>(&kp, &kc);
+        int per_cu = 0;
+        DENGO_HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kp, kBlock, 0));
+        if (per_cu < 1) per_cu = 1;
+        *grid = per_cu * s->cu_count;
+        return 0;
+    }
+
+    // Enqueues the integration on stream.  counter: the persistent schedule's
+    // work counter (one per concurrently running launch); events: bracket the
+    // kernel with the struct's timing events (read by evolve_device).
     static int launch(Data *d, DevState *s, const dengo_options &o, double dtf, double dt0,
                       double z, double *d_input, const double *d_rtol, double *d_atol,
-                      long long dims, const dengo_cell_outputs &out, hipStream_t stream)
+                      long long dims, const dengo_cell_outputs &out, hipStream_t stream,
+                      unsigned long long *counter, bool events)
     {
         KParams<Net> p;
         p.input = d_input;
@@ -433,25 +480,20 @@ struct Host {
         p.dt_out = out.dt;
         p.ttot_out = out.ttot;
         p.T_out = out.T;
-        p.counter = s->d_counter;
+        p.counter = counter;
         if (dims <= 0) return 0;
         void (*kp)(KParams<Net>) = nullptr;
         void (*kc)(KParams<Net>) = nullptr;
         if (o.precision == DENGO_FP64) kernels<kFaithful>(&kp, &kc);
         else if (o.precision == DENGO_FP64_FAST) kernels<kFast64>(&kp, &kc);
         else kernels<kMixed>(&kp, &kc);
-        DENGO_HIP_OK(hipEventRecord(s->ev0, stream));
+        if (events) DENGO_HIP_OK(hipEventRecord(s->ev0, stream));
         if (o.schedule == DENGO_PERSISTENT) {
             int grid = o.grid;
-            if (grid <= 0) {
-                int per_cu = 0;
-                DENGO_HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kp, kBlock, 0));
-                if (per_cu < 1) per_cu = 1;
-                grid = per_cu * s->cu_count;
-            }
+            if (grid <= 0 && full_grid(s, o, &grid)) return -1;
             const long long need = (dims + kBlock - 1) / kBlock;
             if ((long long)grid > need) grid = (int)need;
-            hipLaunchKernelGGL(k_reset_counter<0>, dim3(1), dim3(1), 0, stream, s->d_counter);
+            hipLaunchKernelGGL(k_reset_counter<0>, dim3(1), dim3(1), 0, stream, counter);
             DENGO_HIP_OK(hipGetLastError());
             hipLaunchKernelGGL(kp, dim3(grid), dim3(kBlock), 0, stream, p);
         } else {
@@ -460,7 +502,7 @@ struct Host {
             hipLaunchKernelGGL(kc, dim3((unsigned)blocks), dim3(kBlock), 0, stream, p);
         }
         DENGO_HIP_OK(hipGetLastError());
-        DENGO_HIP_OK(hipEventRecord(s->ev1, stream));
+        if (events) DENGO_HIP_OK(hipEventRecord(s->ev1, stream));
         return 0;
     }
 
@@ -485,7 +527,8 @@ struct Host {
             if (!out.dt) out.dt = s->d_dt;
         }
         d->current_z = z;
-        if (launch(d, s, o, dtf, dt0, z, d_input, d_rtol, d_atol, dims, out, st)) return -1;
+        if (launch(d, s, o, dtf, dt0, z, d_input, d_rtol, d_atol, dims, out, st, s->d_counter, true))
+            return -1;
         if (ttot_min) {
             if (dims > 0) {
                 hipLaunchKernelGGL(k_min_ttot<0>, dim3(1), dim3(1024), 0, st, out.ttot, out.dt,
@@ -594,6 +637,36 @@ struct Host {
         long long nneg = 0;
         for (long long i = 0; i < dims; ++i) nneg += neg[i];
         return nneg;
+    }
+
+    // Hooks of the patch server (dengo_server.hip).
+    static int hook_prepare(void *data, const dengo_options *o, int *full)
+    {
+        Data *d = (Data *)data;
+        dengo_options oo;
+        if (check_options(o, &oo)) return -1;
+        DevState *s = state(d);
+        if (ensure_device(d, s)) return -1;
+        return full_grid(s, oo, full);
+    }
+
+    static int hook_launch(void *data, const LaunchArgs &a)
+    {
+        Data *d = (Data *)data;
+        dengo_options o;
+        if (check_options(&a.opt, &o)) return -1;
+        return launch(d, state(d), o, a.dtf, a.dt0, a.z, a.x, a.rtol, a.atol, a.n, a.out,
+                      a.stream, a.counter, false);
+    }
+
+    static NetHooks hooks()
+    {
+        NetHooks h;
+        h.nspecies = N;
+        h.isothermal = 0;
+        h.prepare = hook_prepare;
+        h.launch = hook_launch;
+        return h;
     }
 
     static int calculate_rhs(double *input, double *rhs, int nstrip, int nchem, void *sdata)

@@ -122,12 +122,16 @@ void kernel_for(bool persist, void (**k)(B::NetDev, double, double, const double
     else *k = B::k_big<MODE, false>;
 }
 
+using KFn = void (*)(B::NetDev, double, double, const double *, double *, const double *,
+                     double *, long long, dengo_cell_outputs, int, unsigned long long *);
+
+// Enqueues the integration on stream; counter is the persistent schedule's
+// work counter, events brackets the kernel with the struct's timing events.
 int launch(synthetic_data *d, const dengo_options &o, double dtf, double dt0, const double *d_T,
            double *d_x, const double *d_rtol, double *d_atol, long long dims,
-           const dengo_cell_outputs &out, hipStream_t stream)
+           const dengo_cell_outputs &out, hipStream_t stream, unsigned long long *counter,
+           bool events)
 {
-    using KFn = void (*)(B::NetDev, double, double, const double *, double *, const double *,
-                         double *, long long, dengo_cell_outputs, int, unsigned long long *);
     KFn k = nullptr;
     const bool persist = o.schedule == DENGO_PERSISTENT;
     const int mode = o.precision == DENGO_FP64 ? B::kLU : B::kGJ;
@@ -147,15 +151,15 @@ int launch(synthetic_data *d, const dengo_options &o, double dtf, double dt0, co
         grid = dims;
         if (grid > 0x7fffffffLL) return set_error("too many cells: %lld", dims);
     }
-    DENGO_HIP_OK(hipEventRecord(d->ev0, stream));
+    if (events) DENGO_HIP_OK(hipEventRecord(d->ev0, stream));
     if (persist) {
-        hipLaunchKernelGGL(k_reset_counter<0>, dim3(1), dim3(1), 0, stream, d->d_counter);
+        hipLaunchKernelGGL(k_reset_counter<0>, dim3(1), dim3(1), 0, stream, counter);
         DENGO_HIP_OK(hipGetLastError());
     }
     hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(B::NTHR), 0, stream, net_view(d), dtf, dt0,
-                       d_T, d_x, d_rtol, d_atol, dims, out, o.max_substeps, d->d_counter);
+                       d_T, d_x, d_rtol, d_atol, dims, out, o.max_substeps, counter);
     DENGO_HIP_OK(hipGetLastError());
-    DENGO_HIP_OK(hipEventRecord(d->ev1, stream));
+    if (events) DENGO_HIP_OK(hipEventRecord(d->ev1, stream));
     return 0;
 }
 
@@ -173,6 +177,32 @@ int check_opts(const dengo_options *opt, dengo_options *o)
     return 0;
 }
 
+
+int hook_prepare(void *data, const dengo_options *opt, int *full)
+{
+    synthetic_data *d = (synthetic_data *)data;
+    dengo_options o;
+    if (check_opts(opt, &o) || ensure_device(d)) return -1;
+    KFn k = nullptr;
+    const bool persist = true;
+    if (o.precision == DENGO_FP64) kernel_for<B::kLU>(persist, &k);
+    else kernel_for<B::kGJ>(persist, &k);
+    int per_cu = 0;
+    DENGO_HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, B::NTHR, 0));
+    if (per_cu < 1) per_cu = 1;
+    *full = per_cu * d->cu_count;
+    return 0;
+}
+
+int hook_launch(void *data, const LaunchArgs &a)
+{
+    synthetic_data *d = (synthetic_data *)data;
+    dengo_options o;
+    if (check_opts(&a.opt, &o)) return -1;
+    if (!a.T) return set_error("the dense network needs per-cell temperatures");
+    return launch(d, o, a.dtf, a.dt0, a.T, a.x, a.rtol, a.atol, a.n, a.out, a.stream, a.counter,
+                  false);
+}
 }  // namespace
 
 extern "C" {
@@ -243,7 +273,9 @@ int dengo_synthetic_evolve_device(synthetic_data *d, const dengo_options *opt, d
         if (!out.ttot) out.ttot = d->d_ttot;
         if (!out.dt) out.dt = d->d_dt;
     }
-    if (dims > 0 && launch(d, o, dtf, dt0, d_T, d_input, d_rtol, d_atol, dims, out, st)) return -1;
+    if (dims > 0 && launch(d, o, dtf, dt0, d_T, d_input, d_rtol, d_atol, dims, out, st,
+                           d->d_counter, true))
+        return -1;
     if (ttot_min) {
         if (dims > 0) {
             hipLaunchKernelGGL(k_min_ttot<0>, dim3(1), dim3(1024), 0, st, out.ttot, out.dt, dims,
@@ -354,3 +386,13 @@ long long dengo_synthetic_eval(synthetic_data *d, const double *T, const double 
 }
 
 }  // extern "C"
+
+NetHooks dengo::hooks_synthetic()
+{
+    NetHooks h;
+    h.nspecies = B::N;
+    h.isothermal = 1;
+    h.prepare = hook_prepare;
+    h.launch = hook_launch;
+    return h;
+}

@@ -37,6 +37,34 @@ class CellOutputs(ctypes.Structure):
                 ("dt", ctypes.c_void_p), ("ttot", ctypes.c_void_p), ("T", ctypes.c_void_p)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_size_t), ("slots", ctypes.c_int),
+                ("grid_share", ctypes.c_int), ("max_bypass", ctypes.c_int),
+                ("reserved", ctypes.c_int * 5)]
+
+
+class PatchResult(ctypes.Structure):
+    _fields_ = [("ticket", ctypes.c_longlong), ("network", ctypes.c_int), ("error", ctypes.c_int),
+                ("ncells", ctypes.c_longlong), ("slot", ctypes.c_int),
+                ("inflight_at_admit", ctypes.c_int), ("max_inflight", ctypes.c_int),
+                ("status_counts", ctypes.c_int * 3), ("t_submit", ctypes.c_double),
+                ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double),
+                ("latency_ms", ctypes.c_double), ("queue_ms", ctypes.c_double),
+                ("dev_start_ms", ctypes.c_double), ("dev_end_ms", ctypes.c_double),
+                ("message", ctypes.c_char * 160)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_longlong), ("completed", ctypes.c_longlong),
+                ("failed", ctypes.c_longlong), ("waiting", ctypes.c_longlong),
+                ("inflight", ctypes.c_longlong), ("cells_completed", ctypes.c_longlong),
+                ("arena_bytes", ctypes.c_size_t), ("arena_used", ctypes.c_size_t),
+                ("arena_peak", ctypes.c_size_t), ("slots", ctypes.c_int),
+                ("max_inflight_seen", ctypes.c_int), ("first_submit", ctypes.c_double),
+                ("last_done", ctypes.c_double)]
+
+
+NET_PRIMORDIAL, NET_OXYGEN, NET_SYNTHETIC = 0, 1, 2
 NETWORKS = ("primordial", "oxygen")
 
 
@@ -97,6 +125,21 @@ SIGNATURES.update({
                           _dp, _dp, _dp, ctypes.c_longlong, _ip, _ip, _ip, _ip, _dp]),
     "dengo_synthetic_eval": (ctypes.c_longlong, [ctypes.c_void_p, _dp, _dp, ctypes.c_longlong,
                                                  _dp, _dp]),
+    "dengo_server_default_config": (None, [ctypes.POINTER(ServerConfig)]),
+    "dengo_server_create": (ctypes.c_void_p, [ctypes.POINTER(ServerConfig), ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
+    "dengo_server_submit": (ctypes.c_longlong, [ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong,
+                                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                                ctypes.c_double, ctypes.c_double]),
+    "dengo_server_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PatchResult),
+                                         ctypes.c_int]),
+    "dengo_server_drain": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    "dengo_server_get_stats": (None, [ctypes.c_void_p, ctypes.POINTER(ServerStats)]),
+    "dengo_server_now": (ctypes.c_double, [ctypes.c_void_p]),
+    "dengo_server_destroy": (None, [ctypes.c_void_p]),
+    "dengo_host_alloc": (ctypes.c_void_p, [ctypes.c_size_t]),
+    "dengo_host_free": (None, [ctypes.c_void_p]),
 })
 
 
@@ -376,3 +419,108 @@ class SyntheticSolver(object):
         if r < 0:
             raise RuntimeError("eval failed: %s" % last_error())
         return f, J, int(r)
+
+
+class PinnedArray(object):
+    """A numpy array in pinned host memory (dengo_host_alloc), so that the
+    server's copies are asynchronous."""
+
+    def __init__(self, shape, dtype):
+        dtype = np.dtype(dtype)
+        nbytes = int(np.prod(shape)) * dtype.itemsize
+        self._ptr = load().dengo_host_alloc(max(nbytes, 1))
+        if not self._ptr:
+            raise MemoryError(last_error())
+        buf = (ctypes.c_char * max(nbytes, 1)).from_address(self._ptr)
+        self.array = np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
+
+    def free(self):
+        if self._ptr:
+            self.array = None
+            load().dengo_host_free(self._ptr)
+            self._ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class PatchServer(object):
+    """Continuous batching of patch solves on one GPU (include/dengo_server.h).
+
+    Networks are given as solver objects (PrimordialSolver, OxygenSolver,
+    SyntheticSolver); submit() queues a patch of host arrays, which must stay
+    alive until its completion record is polled (the server keeps a
+    reference)."""
+
+    def __init__(self, primordial=None, oxygen=None, synthetic=None, precision=MIXED,
+                 hbm_budget=8 << 30, slots=4, grid_share=0, max_bypass=8):
+        lib = load()
+        cfg = ServerConfig()
+        lib.dengo_server_default_config(ctypes.byref(cfg))
+        cfg.hbm_budget = int(hbm_budget)
+        cfg.slots = slots
+        cfg.grid_share = grid_share
+        cfg.max_bypass = max_bypass
+        self._solvers = (primordial, oxygen, synthetic)
+        ptr = [None if x is None else x.data for x in self._solvers]
+        self.srv = lib.dengo_server_create(ctypes.byref(cfg), ptr[0], ptr[1], ptr[2], precision)
+        if not self.srv:
+            raise RuntimeError("dengo_server_create failed: %s" % last_error())
+        self._keep = {}
+
+    def submit(self, network, state, rtol, atol, dtf, T=None, status=None, dt0=-1.0, z=-1.0):
+        """Queue one patch; state and atol [n, N] are updated in place at
+        completion, status[n] (int32) receives per-cell status.  Returns the
+        ticket."""
+        arrays = [state, rtol, atol] + [a for a in (T, status) if a is not None]
+        for a in arrays:
+            if not a.flags["C_CONTIGUOUS"]:
+                raise ValueError("patch arrays must be C-contiguous")
+        n = state.shape[0]
+        t = load().dengo_server_submit(
+            self.srv, network, n, state.ctypes.data, rtol.ctypes.data, atol.ctypes.data,
+            None if T is None else T.ctypes.data, None if status is None else status.ctypes.data,
+            dtf, dt0, z)
+        if t < 0:
+            raise RuntimeError("dengo_server_submit failed: %s" % last_error())
+        self._keep[t] = arrays
+        return t
+
+    def poll(self, max_records=256):
+        buf = (PatchResult * max_records)()
+        k = load().dengo_server_poll(self.srv, buf, max_records)
+        out = []
+        for i in range(k):
+            r = buf[i]
+            rec = {f: getattr(r, f) for f, _ in PatchResult._fields_ if f != "message"}
+            rec["status_counts"] = list(r.status_counts)
+            rec["message"] = r.message.decode(errors="replace")
+            self._keep.pop(r.ticket, None)
+            out.append(rec)
+        return out
+
+    def drain(self, timeout_s=600.0):
+        _check(load().dengo_server_drain(self.srv, timeout_s), "dengo_server_drain")
+
+    def stats(self):
+        st = ServerStats()
+        load().dengo_server_get_stats(self.srv, ctypes.byref(st))
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def now(self):
+        return load().dengo_server_now(self.srv)
+
+    def close(self):
+        if getattr(self, "srv", None):
+            load().dengo_server_destroy(self.srv)
+            self.srv = None
+            self._keep = {}
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -22,7 +22,7 @@ def declared_functions():
             line = text[line_start:m.start()]
             if line.lstrip().startswith("#") or name in ("X", "sizeof", "if", "return"):
                 continue
-            if re.search(r"(void|int|double|long long|char|_data)\s*\**\s*$", line):
+            if re.search(r"(void|int|double|long long|char|_data|dengo_server)\s*\**\s*$", line):
                 names.add(name)
     return names
 
