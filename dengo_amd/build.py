@@ -48,5 +48,41 @@ def build(verbose=False, extra=()):
     return LIB
 
 
+def build_solver(output_dir, solver_name, verbose=False):
+    """Compiles the solver ChemicalNetwork.write_solver wrote into output_dir
+    (<name>_solver.hip and its headers) with this package's kernels into
+    output_dir/lib<name>_solver.so."""
+    src = os.path.join(output_dir, "%s_solver.hip" % solver_name)
+    if not os.path.exists(src):
+        raise RuntimeError("%s not found: run write_solver first" % src)
+    lib = os.path.join(output_dir, "lib%s_solver.so" % solver_name)
+    cmd = [hipcc()] + HIPCC_FLAGS + ["-I", output_dir, "-I", CSRC, "-I", os.path.join(ROOT, "include"),
+                                     "-o", lib + ".tmp", src, os.path.join(CSRC, "dengo_common.hip")]
+    if verbose:
+        print(" ".join(cmd))
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed (%d)" % r.returncode)
+    os.replace(lib + ".tmp", lib)
+    return lib
+
+
+GENERATED = os.path.join(ROOT, "tests", "generated")
+
+
+def build_generated_examples():
+    """Writes (ChemicalNetwork.write_solver) and builds the generated solvers
+    the tests load: carbon, an ion-by-ion network with no hand-written code
+    in this package (tests/generated/carbon)."""
+    if ROOT not in sys.path:
+        sys.path.insert(0, ROOT)
+    from dengo_amd.networks import ion_network
+    out = os.path.join(GENERATED, "carbon")
+    net = ion_network("C", seed=9)
+    net.write_solver("carbon", output_dir=out)
+    return [build_solver(out, "carbon")]
+
+
 if __name__ == "__main__":
     print(build(verbose=True))

@@ -232,13 +232,15 @@ class ChemicalNetwork(object):
 
     # -- writer (dengo/chemical_network.py:258-328) --------------------------
     def write_tables(self, solver_name, output_dir="."):
-        """Write ``<name>_tables.bin`` (FP64 [ntables][len(T)]) and its JSON
+        """Write ``<name>_tables.bin`` (the DENGOTB1 table file of
+        dengo_amd.tables, read by ``<name>_setup_data``) and its JSON
         manifest.  Returns the blob path."""
         if not os.path.isdir(output_dir):
             os.makedirs(output_dir)
+        from .tables import write_blob
         block = self.table_block()
         blob = os.path.join(output_dir, "%s_tables.bin" % solver_name)
-        block.astype("<f8").tofile(blob)
+        write_blob(blob, block, self.T_bounds, self.z_bounds)
         manifest = {
             "solver_name": solver_name,
             "tables": self.table_names(),
@@ -249,6 +251,7 @@ class ChemicalNetwork(object):
             "species": self.species_list(),
             "weights": [float(s.weight) for s in self.ordered_species()],
             "dtype": "<f8",
+            "format": "DENGOTB1: 64-byte header, then FP64 [tables][n_T]",
         }
         with open(os.path.join(output_dir, "%s_tables.json" % solver_name), "w") as f:
             json.dump(manifest, f, indent=1)

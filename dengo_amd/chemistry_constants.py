@@ -17,4 +17,5 @@ GAMMA = 5.0 / 3.0
 GAMMA_H2 = 7.0 / 5.0
 
 # Atomic weights (dengo/periodictabledump.csv:1,2,8).
-ATOMIC_WEIGHT = {"H": 1.00794, "He": 4.002602, "O": 15.9994}
+# dengo/periodictabledump.csv:1-2, 6-8
+ATOMIC_WEIGHT = {"H": 1.00794, "He": 4.002602, "C": 12.0107, "N": 14.0067, "O": 15.9994}

@@ -132,6 +132,24 @@ def render_data_header(net, solver_name):
     w("   of cells whose RHS saw a negative value, or a negative error code. */")
     w("long long dengo_%s_eval(%s_data *data, const double *u, long long dims, double z," % (n, n))
     w("        double *f, double *J, double *T, double *rates);")
+    w("/* BE_chem_solve(calculate_rhs_%s, calculate_jacobian_%s, ...) as the Cython" % (n, n))
+    w("   harness calls it (dengo/solvers/BE_chem_solve.C:79-241,")
+    w("   rates_and_rate_tables_run.pyx.template:142) on host arrays: one backward-Euler")
+    w("   step of nstrip cells that share dt and the Newton convergence decision; u and")
+    w("   atol are scaled and unscaled in place as the reference does; the redshift is")
+    w("   data->current_z.  Returns 0 (converged), 1 (not converged in 10 sweeps, or a")
+    w("   failed RHS/Jacobian/NaN update), or a negative error code. */")
+    w("int dengo_%s_be_chem_solve(%s_data *data, double *u, double dt, double *rtol," % (n, n))
+    w("        double *atol, int nstrip, int nchem, double *scaling);")
+    w("/* ensure_electron_consistency (rates_and_rate_tables.C.template:604-630): de of")
+    w("   cell i becomes the free-electron total of cells 0..i (the reference's total")
+    w("   runs across the strip). */")
+    w("void ensure_electron_consistency_%s(double *input, int nstrip, int nchem);" % n)
+    w("/* Sets data->current_z, the redshift the strip solve uses. */")
+    w("void dengo_%s_set_current_z(%s_data *data, double z);" % (n, n))
+    w("/* Copies data->Ts (temperatures of the last strip RHS evaluation) for")
+    w("   min(nstrip, MAX_NCELLS) cells; returns that count. */")
+    w("int dengo_%s_strip_temperatures(const %s_data *data, double *T, int nstrip);" % (n, n))
     w("#ifdef __cplusplus")
     w("}")
     w("static inline double dengo_evolve_%s(double dtf, double &dt, double z, double *input," % solver_name)
@@ -145,19 +163,143 @@ def render_data_header(net, solver_name):
     return "\n".join(L) + "\n"
 
 
-BUILTIN_KERNELS = {"primordial"}
+PRIMORDIAL_SPECIES = ("H2I", "H2II", "HI", "HII", "HM", "HeI", "HeII", "HeIII", "de", "ge")
+
+
+def device_network_kind(net):
+    """'prim' for the primordial species set (the hand-written device network
+    PrimNet, whose cooling terms are not monomials), 'ma' for networks the
+    generic mass-action device network MANet evaluates from a descriptor."""
+    return "prim" if tuple(net.species_list()) == PRIMORDIAL_SPECIES else "ma"
+
+
+def render_instantiation(net, solver_name, kind=None):
+    """<name>_solver.hip: the kernels of dengo_solver.h instantiated on the
+    network's device form, the host traits of <name>_data and the C ABI the
+    header declares (dengo/templates/rates_and_rate_tables.C.template
+    counterparts), plus the patch-server hooks."""
+    kind = kind or device_network_kind(net)
+    n = solver_name
+    N = n.upper()
+    species = " ".join(net.species_list())
+    if kind == "prim":
+        net_type = "dengo::PrimNet"
+        includes = ['#include "dengo_solver.h"', '#include "%s_solver.h"' % n]
+        what = "the hand-written device network (dengo_device.h)"
+    else:
+        net_type = "dengo::MANet<dengo::net_%s::Desc>" % n
+        includes = ['#include "dengo_solver.h"', '#include "net_%s.h"' % n,
+                    '#include "%s_solver.h"' % n]
+        what = "the generated descriptor net_%s.h" % n
+    L = ["// Generated by dengo_amd.codegen: do not edit.",
+         "// C ABI of the %s network (%s): kernels from" % (n, species),
+         "// dengo_solver.h instantiated on %s, host struct" % what,
+         "// from %s_solver.h." % n]
+    L += includes
+    L += ["",
+          "namespace {",
+          "struct Traits {",
+          "    using Data = %s_data;" % n,
+          "    static constexpr int NSPEC = %s_NSPECIES;" % N,
+          "    static constexpr int NTABLES = %s_NTABLES;" % N,
+          "    static constexpr int NT = %s_NT;" % N,
+          "    static const size_t *table_offsets()",
+          "    {",
+          "        static const size_t o[] = {",
+          "#define X(f) offsetof(%s_data, f)," % n,
+          "            %s_TABLE_FIELDS(X)" % N,
+          "#undef X",
+          "        };",
+          "        return o;",
+          "    }",
+          "    static const char *const *field_names()",
+          "    {",
+          "        static const char *const f[] = %s_FIELD_NAMES;" % N,
+          "        return f;",
+          "    }",
+          "};",
+          "using NetHost = dengo::Host<%s, Traits>;" % net_type,
+          "}  // namespace",
+          "",
+          "DENGO_DEFINE_C_ABI(%s, NetHost)" % n,
+          "",
+          "namespace dengo {",
+          "NetHooks hooks_%s() { return NetHost::hooks(); }" % n,
+          "}  // namespace dengo",
+          ""]
+    return "\n".join(L)
+
+
+def render_run_harness(net, solver_name):
+    """<name>_run.py: ctypes harness of the built solver with the reference's
+    run_<name>(ics, tf, niter, intermediate, z) -> (rv, rv_t) contract
+    (rates_and_rate_tables_run.pyx.template:81-205)."""
+    n = solver_name
+    species = net.species_list()
+    weights = {s.name: float(s.weight) for s in net.ordered_species()}
+    L = ['"""ctypes harness of the %s solver (generated by dengo_amd.codegen; do not edit).' % n,
+         "",
+         "Build the library first: dengo_amd.build.build_solver(<this directory>, %r)." % n,
+         "run_%s(ics, tf, niter, intermediate, z) -> (rv, rv_t) follows" % n,
+         "dengo/templates/rates_and_rate_tables_run.pyx.template:81-205.",
+         '"""',
+         "import os",
+         "",
+         "from dengo_amd import harness, runtime",
+         "from dengo_amd.tables import read_blob",
+         "",
+         "HERE = os.path.dirname(os.path.abspath(__file__))",
+         "SOLVER_NAME = %r" % n,
+         "SPECIES = %r" % (species,),
+         "WEIGHTS = %r" % (weights,),
+         "ENERGY = %r" % net.energy_term.name,
+         "LIBRARY = os.path.join(HERE, 'lib%s_solver.so')" % n,
+         "TABLES = os.path.join(HERE, '%s_tables.bin')" % n,
+         "",
+         "",
+         "def load():",
+         '    """The solver with the tables written beside this file."""',
+         "    tables, T_bounds, _ = read_blob(TABLES)",
+         "    return runtime.GeneratedSolver(LIBRARY, SOLVER_NAME, len(SPECIES), tables, T_bounds)",
+         "",
+         "",
+         "def run_%s(ics, tf, niter=10000, intermediate=1, z=-1.0, solver=None):" % n,
+         "    own = solver is None",
+         "    solver = solver or load()",
+         "    try:",
+         "        return harness.run_network(solver, SPECIES, WEIGHTS, ENERGY, ics, tf, niter,",
+         "                                   intermediate, z)",
+         "    finally:",
+         "        if own:",
+         "            solver.close()",
+         ""]
+    return "\n".join(L)
 
 
 def render_solver(net, solver_name, solver_template, ode_solver_source, output_dir,
                   main_name="main"):
+    """The template set of this backend: <name>_solver.h (drop-in header),
+    net_<name>.h (device descriptor, mass-action networks), <name>_solver.hip
+    (kernel instantiation and C ABI) and <name>_run.py (ctypes harness);
+    build with dengo_amd.build.build_solver(output_dir, name)."""
     if solver_template != "rates_and_rate_tables_hip":
         raise ValueError("unknown solver_template %r: this backend ships "
                          "'rates_and_rate_tables_hip'" % solver_template)
+    if ode_solver_source not in ("BE_chem_solve_hip.hip", "BE_chem_solve.C"):
+        raise ValueError("unknown ode_solver_source %r: the backward-Euler solver is "
+                         "'BE_chem_solve_hip.hip'" % ode_solver_source)
+    kind = device_network_kind(net)
+    files = [("%s_solver.h" % solver_name, render_data_header(net, solver_name))]
+    if kind == "ma":
+        files.append(("net_%s.h" % solver_name, render_device_network(net, solver_name)))
+    files.append(("%s_solver.hip" % solver_name, render_instantiation(net, solver_name, kind)))
+    files.append(("%s_run.py" % solver_name, render_run_harness(net, solver_name)))
     written = []
-    path = os.path.join(output_dir, "%s_solver.h" % solver_name)
-    with open(path, "w") as f:
-        f.write(render_data_header(net, solver_name))
-    written.append(path)
+    for name, text in files:
+        path = os.path.join(output_dir, name)
+        with open(path, "w") as f:
+            f.write(text)
+        written.append(path)
     return written
 
 

@@ -24,7 +24,7 @@ struct PrimNet {
     static constexpr int DE = prim::DE;
     using Env = prim::Env;
     static DENGO_D double weight(int k) { return prim::W[k]; }
-    static DENGO_D double free_e(int k) { return prim::FREE_E[k]; }
+    static DENGO_HD double free_e(int k) { return prim::FREE_E[k]; }
     static DENGO_D bool rhs(const double (&u)[N], double (&f)[N], double (&rs)[NTAB],
                             double &T, const Env &env)
     {
@@ -118,9 +118,13 @@ DENGO_D void begin_cell(Cell<Net> &c, long long id, const KParams<Net> &p)
     }
 }
 
-// One Newton sweep in reference-faithful FP64.
+// Why a faithful sweep failed: its RHS saw a negative species, its Jacobian
+// did, or its update produced a NaN before the first unconverged entry.
+enum FailKind { FK_NONE = 0, FK_RHS = 1, FK_JAC = 2, FK_NAN = 3 };
+
+// One Newton sweep in reference-faithful FP64; on SW_FAILED, kind says why.
 template <class Net>
-DENGO_D int sweep_fp64(Cell<Net> &c, const KParams<Net> &p)
+DENGO_D int sweep_fp64(Cell<Net> &c, const KParams<Net> &p, int &kind)
 {
     constexpr int N = Net::N;
     double f[N], rs[Net::NTAB], J[N * N];
@@ -129,9 +133,13 @@ DENGO_D int sweep_fp64(Cell<Net> &c, const KParams<Net> &p)
     // BE_Resid_Fun: g(u) = u - u0 - dt f(u), scaled
 #pragma unroll
     for (int i = 0; i < N; ++i) c.u[i] *= c.sc[i];
+    kind = FK_NONE;
     const bool ok = Net::rhs(c.u, f, rs, T, p.env);
     c.T = T;
-    if (!ok) return SW_FAILED;
+    if (!ok) {
+        kind = FK_RHS;
+        return SW_FAILED;
+    }
 #pragma unroll
     for (int i = 0; i < N; ++i) c.u[i] *= c.inv[i];
 #pragma unroll
@@ -144,7 +152,10 @@ DENGO_D int sweep_fp64(Cell<Net> &c, const KParams<Net> &p)
     bool neg = false;
 #pragma unroll
     for (int i = 0; i < N; ++i) neg = neg || (c.u[i] < 0.0);
-    if (neg) return SW_FAILED;
+    if (neg) {
+        kind = FK_JAC;
+        return SW_FAILED;
+    }
     Net::jacobian(c.u, rs, T, p.env, J);
 #pragma unroll
     for (int i = 0; i < N; ++i) c.u[i] *= c.inv[i];
@@ -174,10 +185,20 @@ DENGO_D int sweep_fp64(Cell<Net> &c, const KParams<Net> &p)
             else if (c.u[i] != c.u[i]) nan = true;
         }
     }
-    if (nan) return SW_FAILED;
+    if (nan) {
+        kind = FK_NAN;
+        return SW_FAILED;
+    }
     if (!unsolved) return SW_CONVERGED;
     c.sweep++;
     return c.sweep == 10 ? SW_FAILED : SW_CONTINUE;
+}
+
+template <class Net>
+DENGO_D int sweep_fp64(Cell<Net> &c, const KParams<Net> &p)
+{
+    int kind;
+    return sweep_fp64(c, p, kind);
 }
 
 // BE_chem_solve exit + one pass of dengo_evolve's substep loop.  Returns true

@@ -324,8 +324,15 @@ DENGO_D void begin_be_fast(Cell<Net> &c)
 #ifndef DENGO_REFINE_STEPS
 #define DENGO_REFINE_STEPS 3
 #endif
+#ifndef DENGO_REFINE_TOL
+#define DENGO_REFINE_TOL 1e-3
+#endif
+#ifndef DENGO_REFINE_MODE
+#define DENGO_REFINE_MODE 1
+#endif
 constexpr int kRefineSteps = DENGO_REFINE_STEPS;
-constexpr double kRefineTol = 1e-3;
+constexpr int kRefineMode = DENGO_REFINE_MODE;
+constexpr double kRefineTol = DENGO_REFINE_TOL;
 
 template <class Net, class LUT>
 DENGO_D int sweep_fast(Cell<Net> &c, const KParams<Net> &p)
@@ -418,9 +425,21 @@ DENGO_D int sweep_fast(Cell<Net> &c, const KParams<Net> &p)
             for (int i = 0; i < M; ++i) {
                 const double dy = e[i] * (double)y32[i];
                 y[i] = y[i] + dy;
-                small = small && fabs(dy) <= tol[i];
+                if constexpr (kRefineMode == 2) {
+                    const double un = fabs(c.u[i] * c.inv[i] - y[i]);
+                    small = small && fabs(dy) <= fmin(tol[i], kRefineTol * un);
+                } else {
+                    small = small && fabs(dy) <= tol[i];
+                }
             }
             refined = small;
+        }
+        if constexpr (kRefineMode >= 1) {
+            // a component the FP32 step would drive negative is decided by FP64
+            bool neg = false;
+#pragma unroll
+            for (int i = 0; i < M; ++i) neg = neg || (c.u[i] - c.sc[i] * y[i] < 0.0);
+            refined = refined && !neg;
         }
         if (!refined) {
             double rp64[M];

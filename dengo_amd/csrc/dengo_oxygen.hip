@@ -1,12 +1,13 @@
-// C ABI of the oxygen ion-by-ion network (O_1..O_9 de ge): kernels from
-// dengo_solver.h instantiated on the generated descriptor net_oxygen.h, host
-// struct from include/oxygen_solver.h.
+// Generated by dengo_amd.codegen: do not edit.
+// C ABI of the oxygen network (O_1 O_2 O_3 O_4 O_5 O_6 O_7 O_8 O_9 de ge): kernels from
+// dengo_solver.h instantiated on the generated descriptor net_oxygen.h, host struct
+// from oxygen_solver.h.
 #include "dengo_solver.h"
 #include "net_oxygen.h"
 #include "oxygen_solver.h"
 
 namespace {
-struct OxyTraits {
+struct Traits {
     using Data = oxygen_data;
     static constexpr int NSPEC = OXYGEN_NSPECIES;
     static constexpr int NTABLES = OXYGEN_NTABLES;
@@ -26,10 +27,11 @@ struct OxyTraits {
         return f;
     }
 };
-using OxyHost = dengo::Host<dengo::MANet<dengo::net_oxygen::Desc>, OxyTraits>;
+using NetHost = dengo::Host<dengo::MANet<dengo::net_oxygen::Desc>, Traits>;
 }  // namespace
 
-DENGO_DEFINE_C_ABI(oxygen, OxyHost)
+DENGO_DEFINE_C_ABI(oxygen, NetHost)
 
-dengo::NetHooks dengo::hooks_oxygen() { return OxyHost::hooks(); }
-
+namespace dengo {
+NetHooks hooks_oxygen() { return NetHost::hooks(); }
+}  // namespace dengo

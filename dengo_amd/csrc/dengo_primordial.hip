@@ -1,11 +1,12 @@
-// C ABI of the primordial network (H2I H2II HI HII HM HeI HeII HeIII de ge):
-// kernels from dengo_solver.h instantiated on the hand-written device network
-// (dengo_device.h), host struct from include/primordial_solver.h.
+// Generated by dengo_amd.codegen: do not edit.
+// C ABI of the primordial network (H2I H2II HI HII HM HeI HeII HeIII de ge): kernels from
+// dengo_solver.h instantiated on the hand-written device network (dengo_device.h), host struct
+// from primordial_solver.h.
 #include "dengo_solver.h"
 #include "primordial_solver.h"
 
 namespace {
-struct PrimTraits {
+struct Traits {
     using Data = primordial_data;
     static constexpr int NSPEC = PRIMORDIAL_NSPECIES;
     static constexpr int NTABLES = PRIMORDIAL_NTABLES;
@@ -25,10 +26,11 @@ struct PrimTraits {
         return f;
     }
 };
-using PrimHost = dengo::Host<dengo::PrimNet, PrimTraits>;
+using NetHost = dengo::Host<dengo::PrimNet, Traits>;
 }  // namespace
 
-DENGO_DEFINE_C_ABI(primordial, PrimHost)
+DENGO_DEFINE_C_ABI(primordial, NetHost)
 
-dengo::NetHooks dengo::hooks_primordial() { return PrimHost::hooks(); }
-
+namespace dengo {
+NetHooks hooks_primordial() { return NetHost::hooks(); }
+}  // namespace dengo

@@ -129,6 +129,46 @@ __global__ void __launch_bounds__(64) k_eval(const double *u_in, long long n,
     }
 }
 
+// One Newton sweep of BE_chem_solve (BE_chem_solve.C:79-241) for each cell of
+// a strip: lane = cell, scaled state u (read only), the cell's updated state
+// into u_new, code[cell] = 0 converged, 1 not converged, 2 + FailKind on a
+// failed sweep (the host applies the strip-wide consequences), T_out = the
+// temperature of the sweep's RHS evaluation.
+template <class Net>
+__global__ void __launch_bounds__(64) k_strip_sweep(const double *u, double *u_new,
+                                                    const double *u0, const double *sc,
+                                                    const double *inv, const double *at,
+                                                    const double *rt, double dt, long long n,
+                                                    typename Net::Env env, double *T_out,
+                                                    int *code)
+{
+    constexpr int N = Net::N;
+    const long long id = (long long)blockIdx.x * 64 + threadIdx.x;
+    if (id >= n) return;
+    Cell<Net> c;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        c.u[k] = u[id * N + k];
+        c.u0[k] = u0[id * N + k];
+        c.sc[k] = sc[id * N + k];
+        c.inv[k] = inv[id * N + k];
+        c.atol[k] = at[id * N + k];
+        c.rtol[k] = rt[id * N + k];
+    }
+    c.dt = dt;
+    c.sweep = 0;
+    c.nsweep = 0;
+    c.T = 0.0;
+    KParams<Net> p;
+    p.env = env;
+    int kind = FK_NONE;
+    const int r = sweep_fp64(c, p, kind);
+    T_out[id] = c.T;
+#pragma unroll
+    for (int k = 0; k < N; ++k) u_new[id * N + k] = c.u[k];
+    code[id] = r == SW_FAILED ? 2 + kind : (r == SW_CONVERGED ? 0 : 1);
+}
+
 template <int DUMMY>
 __global__ void k_reset_counter(unsigned long long *counter)
 {
@@ -227,6 +267,10 @@ struct DevState {
     double *d_red = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int cu_count = 0;
+    // strip solve (be_chem_solve): initial state, scaling and its inverse
+    long long strip_cap = 0;
+    double *d_u0 = nullptr, *d_sc = nullptr, *d_inv = nullptr, *d_unew = nullptr;
+    int *d_code = nullptr;
 };
 
 inline int check_options(const dengo_options *opt, dengo_options *o)
@@ -423,7 +467,7 @@ struct Host {
         if (s) {
             void *ptrs[] = {s->pairs, s->d_in, s->d_rtol, s->d_atol, s->d_ttot, s->d_dt, s->d_T,
                             s->d_status, s->d_nsub, s->d_nrej, s->d_nsweep, s->d_counter,
-                            s->d_red};
+                            s->d_red, s->d_u0, s->d_sc, s->d_inv, s->d_unew, s->d_code};
             for (void *q : ptrs)
                 if (q) (void)hipFree(q);
             if (s->ev0) (void)hipEventDestroy(s->ev0);
@@ -639,6 +683,119 @@ struct Host {
         return nneg;
     }
 
+    // BE_chem_solve(calculate_rhs, calculate_jacobian, u, dt, rtol, atol, nstrip,
+    // nchem, scaling, data, ...) (BE_chem_solve.C:79-241) for host arrays: the
+    // strip's cells share dt, and the Newton loop runs until every cell passes
+    // the max-norm test (at most 10 sweeps).  Each sweep's per-cell work runs
+    // on the GPU; the host keeps the strip state and applies the reference's
+    // strip-wide outcomes in its order: a negative species in any cell's RHS
+    // (BE_Resid_Fun, :246-275) or Jacobian (BE_Resid_Jac, :280-321) ends the
+    // call with every cell's state left multiplied by the scaling at that
+    // point; a NaN in cell x ends it after cells 0..x are updated; the
+    // scaling loops run on the host in the reference's order.
+    static int be_chem_solve(Data *d, double *u, double dt, double *rtol, double *atol,
+                             int nstrip, int nchem, double *scaling)
+    {
+        if (!d || !u || !rtol || !atol || !scaling) return set_error("null argument");
+        if (nchem != N) return set_error("nchem must be %d", N);
+        if (nstrip <= 0) return 0;
+        DevState *s = state(d);
+        if (ensure_device(d, s) || ensure_scratch(s, nstrip)) return -1;
+        if (nstrip > s->strip_cap) {
+            if (grow(&s->d_u0, (long long)nstrip * N) || grow(&s->d_sc, (long long)nstrip * N) ||
+                grow(&s->d_inv, (long long)nstrip * N) || grow(&s->d_unew, (long long)nstrip * N) ||
+                grow(&s->d_code, (long long)nstrip))
+                return -1;
+            s->strip_cap = nstrip;
+        }
+        const size_t m = (size_t)nstrip * N;
+        std::vector<double> inv(m), unew(m);
+        std::vector<int> code(nstrip);
+        for (size_t i = 0; i < m; ++i) inv[i] = 1.0 / scaling[i];
+        for (size_t i = 0; i < m; ++i) u[i] *= inv[i];
+        for (size_t i = 0; i < m; ++i) atol[i] *= inv[i];
+        const size_t b = m * sizeof(double);
+        int rc = 0;
+        if (hipMemcpy(s->d_u0, u, b, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(s->d_sc, scaling, b, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(s->d_inv, inv.data(), b, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(s->d_atol, atol, b, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(s->d_rtol, rtol, b, hipMemcpyHostToDevice) != hipSuccess)
+            rc = set_error("host-to-device copy failed");
+        const typename Net::Env e = env(d, s, d->current_z);
+        const unsigned blocks = (unsigned)((nstrip + 63) / 64);
+        int rv = 1;
+        bool ended = false;
+        for (int sweep = 0; sweep < 10 && rc == 0 && !ended; ++sweep) {
+            if (hipMemcpy(s->d_in, u, b, hipMemcpyHostToDevice) != hipSuccess) {
+                rc = set_error("host-to-device copy failed");
+                break;
+            }
+            hipLaunchKernelGGL(k_strip_sweep<Net>, dim3(blocks), dim3(64), 0, 0, s->d_in,
+                               s->d_unew, s->d_u0, s->d_sc, s->d_inv, s->d_atol, s->d_rtol, dt,
+                               (long long)nstrip, e, s->d_T, s->d_code);
+            if (hipGetLastError() != hipSuccess ||
+                hipMemcpy(unew.data(), s->d_unew, b, hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(code.data(), s->d_code, nstrip * sizeof(int), hipMemcpyDeviceToHost) !=
+                    hipSuccess) {
+                rc = set_error("strip sweep failed");
+                break;
+            }
+            bool rhs_fail = false, jac_fail = false, unsolved = false;
+            int first_nan = -1;
+            for (int x = 0; x < nstrip; ++x) {
+                rhs_fail = rhs_fail || code[x] == 2 + FK_RHS;
+                jac_fail = jac_fail || code[x] == 2 + FK_JAC;
+                if (first_nan < 0 && code[x] == 2 + FK_NAN) first_nan = x;
+                unsolved = unsolved || code[x] == 1;
+            }
+            if (rhs_fail) {
+                for (size_t i = 0; i < m; ++i) u[i] *= scaling[i];
+                ended = true;
+            } else if (jac_fail) {
+                for (size_t i = 0; i < m; ++i) u[i] = ((u[i] * scaling[i]) * inv[i]) * scaling[i];
+                ended = true;
+            } else if (first_nan >= 0) {
+                // cells 0..first_nan were updated; the rest keep the state after
+                // the residual's and the Jacobian's scaling round trips
+                for (size_t i = 0; i < m; ++i) {
+                    if ((long long)(i / N) <= first_nan) u[i] = unew[i];
+                    else u[i] = (((u[i] * scaling[i]) * inv[i]) * scaling[i]) * inv[i];
+                }
+                ended = true;
+            } else {
+                memcpy(u, unew.data(), b);
+                if (!unsolved) {
+                    rv = 0;
+                    ended = true;
+                }
+            }
+        }
+        const int nts = nstrip < MAX_NCELLS ? nstrip : MAX_NCELLS;
+        if (rc == 0 && hipMemcpy(d->Ts, s->d_T, nts * sizeof(double), hipMemcpyDeviceToHost) !=
+                           hipSuccess)
+            rc = set_error("device-to-host copy failed");
+        for (size_t i = 0; i < m; ++i) u[i] *= scaling[i];
+        for (size_t i = 0; i < m; ++i) atol[i] *= scaling[i];
+        if (rc) return rc;
+        return rv;
+    }
+
+    // ensure_electron_consistency (rates_and_rate_tables.C.template:604-630):
+    // de of cell i is the running total of free electrons over cells 0..i of
+    // the strip (the reference does not reset its total per cell).
+    static void electron_consistency(double *input, int nstrip, int nchem)
+    {
+        if (!input || nchem != N) return;
+        double total_e = 0.0;
+        for (int i = 0; i < nstrip; ++i) {
+            double *c = input + (size_t)i * N;
+            for (int k = 0; k < N; ++k)
+                if (k != Net::GE && k != Net::DE) total_e += c[k] * Net::free_e(k);
+            c[Net::DE] = total_e;
+        }
+    }
+
     // Hooks of the patch server (dengo_server.hip).
     static int hook_prepare(void *data, const dengo_options *o, int *full)
     {
@@ -733,6 +890,26 @@ struct Host {
         if (H::read_blob(#n "_tables.bin", d) != 0) fprintf(stderr, "%s\n", dengo_last_error()); \
     }                                                                                          \
     void n##_free_data(n##_data *d) { H::free_data(d); }                                       \
+    int dengo_##n##_be_chem_solve(n##_data *d, double *u, double dt, double *rt, double *at,   \
+                                  int nstrip, int nchem, double *scaling)                      \
+    {                                                                                          \
+        return H::be_chem_solve(d, u, dt, rt, at, nstrip, nchem, scaling);                     \
+    }                                                                                          \
+    void ensure_electron_consistency_##n(double *input, int nstrip, int nchem)                 \
+    {                                                                                          \
+        H::electron_consistency(input, nstrip, nchem);                                         \
+    }                                                                                          \
+    void dengo_##n##_set_current_z(n##_data *d, double z)                                      \
+    {                                                                                          \
+        if (d) d->current_z = z;                                                               \
+    }                                                                                          \
+    int dengo_##n##_strip_temperatures(const n##_data *d, double *T, int nstrip)               \
+    {                                                                                          \
+        if (!d || !T) return ::dengo::set_error("null argument");                              \
+        const int m = nstrip < MAX_NCELLS ? nstrip : MAX_NCELLS;                               \
+        for (int i = 0; i < m; ++i) T[i] = d->Ts[i];                                           \
+        return m;                                                                              \
+    }                                                                                          \
     int dengo_##n##_evolve_device(n##_data *d, const dengo_options *o, double dtf, double dt0,  \
                                   double z, double *x, const double *rt, double *at,           \
                                   long long dims, const dengo_cell_outputs *out,              \

@@ -65,7 +65,7 @@ struct MANet {
     static constexpr LUPlan<M> plan = make_plan<M>(NZ{});
 
     static DENGO_D double weight(int k) { return D::W[k]; }
-    static DENGO_D double free_e(int k) { return D::FREE_E[k]; }
+    static DENGO_HD double free_e(int k) { return D::FREE_E[k]; }
 
     static DENGO_D double temperature(const double (&u)[N], const TableView &tb)
     {

@@ -94,23 +94,29 @@ def ion_temperature_ge(n, T, weight, ge_index):
     return T * KB_SOLVER * gsum / (density * MH_SOLVER)
 
 
-def oxygen_ics(ncells, seed, nO_range=(1e-4, 1e2), T_range=(1e4, 1e8)):
-    """O I-IX cells: oxygen number density log-uniform, T log-uniform 1e4-1e8 K,
-    ion fractions Dirichlet(1, ..., 1) floored at 1e-20, electrons from charge
-    neutrality, ge from the solver's temperature formula.  Returns
-    (input in mass densities, number densities, T)."""
+def ion_ics(ncells, seed, nions, weight, n_range=(1e-4, 1e2), T_range=(1e4, 1e8)):
+    """Cells of an ion chain with nions ions: element number density
+    log-uniform, T log-uniform, ion fractions Dirichlet(1, ..., 1) floored at
+    1e-20, electrons from charge neutrality, ge from the solver's temperature
+    formula.  Returns (input in mass densities, number densities, T)."""
     rng = np.random.Generator(np.random.PCG64(seed))
-    nO = _logu(rng, nO_range[0], nO_range[1], ncells)
+    nX = _logu(rng, n_range[0], n_range[1], ncells)
     T = _logu(rng, T_range[0], T_range[1], ncells)
-    frac = np.maximum(rng.dirichlet(np.ones(9), ncells), 1e-20)
-    n = np.zeros((ncells, 11))
-    n[:, :9] = frac * nO[:, None]
+    frac = np.maximum(rng.dirichlet(np.ones(nions), ncells), 1e-20)
+    n = np.zeros((ncells, nions + 2))
+    n[:, :nions] = frac * nX[:, None]
     total_e = np.zeros(ncells)
-    for k in range(9):
+    for k in range(nions):
         total_e = total_e + n[:, k] * float(k)
-    n[:, 9] = total_e
-    n[:, 10] = ion_temperature_ge(n, T, OXYGEN_WEIGHT, 10)
+    n[:, nions] = total_e
+    n[:, nions + 1] = ion_temperature_ge(n, T, weight, nions + 1)
     inp = n.copy()
-    inp[:, :10] = n[:, :10] * np.append(np.full(9, OXYGEN_WEIGHT), 1.0)[None, :] * 1.67e-24
+    inp[:, :nions + 1] = (n[:, :nions + 1] *
+                          np.append(np.full(nions, weight), 1.0)[None, :] * 1.67e-24)
     assert np.all(inp > 0) and np.all(np.isfinite(inp))
     return np.ascontiguousarray(inp), n, T
+
+
+def oxygen_ics(ncells, seed, nO_range=(1e-4, 1e2), T_range=(1e4, 1e8)):
+    """O I-IX cells (ion_ics with 9 ions of weight 15.9994)."""
+    return ion_ics(ncells, seed, 9, OXYGEN_WEIGHT, nO_range, T_range)

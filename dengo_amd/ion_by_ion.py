@@ -16,7 +16,7 @@ from .chemistry_constants import ATOMIC_WEIGHT
 from .reaction_classes import (AtomicSpecies, Reaction, CoolingAction, registry_setup,
                                species_registry)
 
-ATOMIC_NUMBER = {"H": 1, "He": 2, "O": 8}
+ATOMIC_NUMBER = {"H": 1, "He": 2, "C": 6, "N": 7, "O": 8}
 
 
 @registry_setup

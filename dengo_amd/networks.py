@@ -48,22 +48,32 @@ def primordial_tables(T_bounds=(1.0, 1e9), n_bins=1024):
 OXYGEN_SPECIES = tuple("O_%d" % k for k in range(1, 10)) + ("de", "ge")
 
 
+def ion_network(atom, seed, T_bounds=(1e4, 1e9), n_bins=1024, stop_time=1e12):
+    """Ion-by-ion network of one element (ions <atom>_1 .. <atom>_{Z+1}, de,
+    ge; Z ionisations, Z recombinations, Z + 1 cooling tables) with smooth
+    synthetic tables on n_bins log-T bins: rates in 1e-16..1e-8 cm^3/s,
+    cooling in 1e-25..1e-21 erg cm^3/s."""
+    from .ion_by_ion import ATOMIC_NUMBER, setup_ionization, synthetic_ion_tables
+    Z = ATOMIC_NUMBER[atom]
+    fns = synthetic_ion_tables(atom, seed)
+    setup_ionization(atom, fns)
+    net = ChemicalNetwork(stop_time=stop_time)
+    for k in range(1, Z + 2):
+        net.add_cooling("%s_%d_c" % (atom, k))
+    for k in range(1, Z + 2):
+        if k < Z + 1:
+            net.add_reaction("%s_%d_i" % (atom, k))
+        if k > 1:
+            net.add_reaction("%s_%d_r" % (atom, k))
+    net.init_temperature(T_bounds, n_bins)
+    return net
+
+
 def oxygen_network(seed=3, T_bounds=(1e4, 1e9), n_bins=1024, stop_time=1e12):
     """O I - O IX ion-by-ion network (11 unknowns, 16 reactions, 9 cooling
     tables) with smooth synthetic tables on 1024 log-T bins over 1e4-1e9 K:
     rates in 1e-16..1e-8 cm^3/s, cooling in 1e-25..1e-21 erg cm^3/s."""
-    from .ion_by_ion import setup_ionization, synthetic_ion_tables
-    fns = synthetic_ion_tables("O", seed)
-    setup_ionization("O", fns)
-    net = ChemicalNetwork(stop_time=stop_time)
-    for k in range(1, 10):
-        net.add_cooling("O_%d_c" % k)
-    for k in range(1, 10):
-        if k < 9:
-            net.add_reaction("O_%d_i" % k)
-        if k > 1:
-            net.add_reaction("O_%d_r" % k)
-    net.init_temperature(T_bounds, n_bins)
+    net = ion_network("O", seed, T_bounds, n_bins, stop_time)
     assert tuple(net.species_list()) == OXYGEN_SPECIES, net.species_list()
     return net
 

@@ -99,15 +99,23 @@ def _network_signatures(n):
                               _ip, _dp, _dp]),
         "dengo_%s_eval" % n: (ctypes.c_longlong, [ctypes.c_void_p, _dp, ctypes.c_longlong,
                                                   ctypes.c_double, _dp, _dp, _dp, _dp]),
+        "dengo_%s_be_chem_solve" % n: (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_double, _dp,
+                                                      _dp, ctypes.c_int, ctypes.c_int, _dp]),
+        "ensure_electron_consistency_%s" % n: (None, [_dp, ctypes.c_int, ctypes.c_int]),
+        "dengo_%s_set_current_z" % n: (None, [ctypes.c_void_p, ctypes.c_double]),
+        "dengo_%s_strip_temperatures" % n: (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int]),
     }
 
 
-# Every symbol include/*.h declares, with its ctypes signature.
-SIGNATURES = {
+COMMON_SIGNATURES = {
     "dengo_default_options": (None, [ctypes.POINTER(Options)]),
     "dengo_last_error": (ctypes.c_char_p, []),
     "dengo_last_kernel_ms": (ctypes.c_double, []),
 }
+
+
+# Every symbol include/*.h declares, with its ctypes signature.
+SIGNATURES = dict(COMMON_SIGNATURES)
 for _n in NETWORKS:
     SIGNATURES.update(_network_signatures(_n))
 SIGNATURES.update({
@@ -166,8 +174,24 @@ def load(path=LIB_PATH):
     return _lib
 
 
-def last_error():
-    return load().dengo_last_error().decode()
+def load_solver_library(path, name):
+    """Loads a solver library written by ChemicalNetwork.write_solver and built
+    by dengo_amd.build.build_solver (the C ABI of one network <name>)."""
+    if not os.path.exists(path):
+        raise RuntimeError("solver library %s not built: run dengo_amd.build.build_solver" % path)
+    load()   # the main library first: one HIP runtime per process (bound through torch)
+    lib = ctypes.CDLL(path)
+    sigs = dict(COMMON_SIGNATURES)
+    sigs.update(_network_signatures(name))
+    for fn_name, (res, args) in sigs.items():
+        fn = getattr(lib, fn_name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def last_error(lib=None):
+    return (lib or load()).dengo_last_error().decode()
 
 
 def _check(rc, what):
@@ -192,29 +216,23 @@ def options(precision=FP64, schedule=PERSISTENT, grid=0, max_substeps=50000,
 
 
 def write_table_blob(path, block, T_bounds, z_bounds=(0.0, 0.0)):
-    """Write the DENGOTB1 blob read by <name>_setup_data: 64-byte header
-    (magic, int32 ntables, int32 nT, double T0, T1, z0, z1, padding) then
-    FP64 [ntables][nT]."""
-    block = np.ascontiguousarray(block, dtype="<f8")
-    hdr = bytearray(64)
-    hdr[0:8] = b"DENGOTB1"
-    hdr[8:16] = np.array([block.shape[0], block.shape[1]], dtype="<i4").tobytes()
-    hdr[16:48] = np.array([T_bounds[0], T_bounds[1], z_bounds[0], z_bounds[1]],
-                          dtype="<f8").tobytes()
-    with open(path, "wb") as f:
-        f.write(bytes(hdr))
-        f.write(block.tobytes())
+    """Write the DENGOTB1 table file read by <name>_setup_data
+    (dengo_amd.tables.write_blob)."""
+    from .tables import write_blob
+    write_blob(path, block, T_bounds, z_bounds)
 
 
 class NetworkSolver(object):
     """One <name>_data struct (a built-in network) with its tables resident on
     the GPU."""
 
-    def __init__(self, name, tables, T_bounds):
-        lib = load()
-        if name not in NETWORKS:
-            raise ValueError("unknown network %r (built in: %s)" % (name, ", ".join(NETWORKS)))
+    def __init__(self, name, tables, T_bounds, lib=None):
+        if lib is None:
+            lib = load()
+            if name not in NETWORKS:
+                raise ValueError("unknown network %r (built in: %s)" % (name, ", ".join(NETWORKS)))
         self.name = name
+        self.lib = lib
         tables = np.ascontiguousarray(tables, dtype=np.float64)
         self.tables = tables
         self.T_bounds = T_bounds
@@ -222,8 +240,10 @@ class NetworkSolver(object):
         self.data = self._fn("%s_alloc_data")()
         if not self.data:
             raise MemoryError("%s_alloc_data" % name)
-        _check(self._fn("%s_set_tables")(self.data, _p(tables), tables.shape[0], tables.shape[1],
-                                         T_bounds[0], T_bounds[1]), "%s_set_tables" % name)
+        rc = self._fn("%s_set_tables")(self.data, _p(tables), tables.shape[0], tables.shape[1],
+                                       T_bounds[0], T_bounds[1])
+        if rc != 0:
+            raise RuntimeError("%s_set_tables failed: %s" % (name, last_error(lib)))
 
     def close(self):
         if getattr(self, "data", None):
@@ -255,7 +275,7 @@ class NetworkSolver(object):
             _p(res["status"], _ip), _p(res["nsub"], _ip), _p(res["nrej"], _ip),
             _p(res["nsweep"], _ip), _p(res["dt_cell"]), _p(res["T"]))
         if ttot < 0:
-            raise RuntimeError("evolve failed: %s" % last_error())
+            raise RuntimeError("evolve failed: %s" % last_error(self.lib))
         res.update(state=out, atol=at, ttot=ttot, dt=dtc.value,
                    kernel_ms=load().dengo_last_kernel_ms())
         return res
@@ -285,8 +305,34 @@ class NetworkSolver(object):
             ctypes.c_void_p(rtol.data_ptr()), ctypes.c_void_p(atol.data_ptr()), n,
             ctypes.byref(co), ctypes.byref(tmin) if sync_min else None,
             ctypes.byref(dmin) if sync_min else None, s)
-        _check(rc, "dengo_%s_evolve_device" % self.name)
+        if rc != 0:
+            raise RuntimeError("dengo_%s_evolve_device failed: %s" % (self.name, last_error(self.lib)))
         return (tmin.value, dmin.value) if sync_min else None
+
+    def be_chem_solve(self, u, dt, rtol, atol, scaling):
+        """One strip backward-Euler step (BE_chem_solve semantics) of the host
+        arrays u, atol [nstrip, N], updated in place; returns 0 or 1."""
+        for a in (u, rtol, atol, scaling):
+            if not (a.dtype == np.float64 and a.flags["C_CONTIGUOUS"] and a.shape == u.shape):
+                raise ValueError("expected C-contiguous float64 arrays of shape %s" % (u.shape,))
+        rc = self._fn("dengo_%s_be_chem_solve")(self.data, _p(u), dt, _p(rtol), _p(atol),
+                                                u.shape[0], u.shape[1], _p(scaling))
+        if rc < 0:
+            raise RuntimeError("be_chem_solve failed: %s" % last_error(self.lib))
+        return rc
+
+    def ensure_electron_consistency(self, u):
+        if not (u.dtype == np.float64 and u.flags["C_CONTIGUOUS"]):
+            raise ValueError("expected a C-contiguous float64 array")
+        self._fn("ensure_electron_consistency_%s")(_p(u), u.shape[0], u.shape[1])
+
+    def set_current_z(self, z):
+        self._fn("dengo_%s_set_current_z")(self.data, z)
+
+    def strip_temperatures(self, nstrip):
+        T = np.zeros(nstrip)
+        m = self._fn("dengo_%s_strip_temperatures")(self.data, _p(T), nstrip)
+        return T[:m]
 
     def eval(self, u, z=-1.0, jacobian=True):
         """RHS f[n,N], Jacobian J[n,N,N] (J[c, col, row]), T[n], rates[n,NTAB]
@@ -300,8 +346,18 @@ class NetworkSolver(object):
         rates = np.zeros((n, self.tables.shape[0]))
         r = self._fn("dengo_%s_eval")(self.data, _p(u), n, z, _p(f), _p(J), _p(T), _p(rates))
         if r < 0:
-            raise RuntimeError("eval failed: %s" % last_error())
+            raise RuntimeError("eval failed: %s" % last_error(self.lib))
         return f, J, T, rates, int(r)
+
+
+class GeneratedSolver(NetworkSolver):
+    """A network solver built from ChemicalNetwork.write_solver output:
+    library lib<name>_solver.so and the table blob <name>_tables.bin."""
+
+    def __init__(self, lib_path, name, nspecies, tables, T_bounds):
+        self.N = nspecies
+        NetworkSolver.__init__(self, name, tables, T_bounds,
+                               lib=load_solver_library(lib_path, name))
 
 
 class PrimordialSolver(NetworkSolver):

@@ -263,6 +263,24 @@ double dengo_primordial_evolve_host(primordial_data *data, const dengo_options *
    of cells whose RHS saw a negative value, or a negative error code. */
 long long dengo_primordial_eval(primordial_data *data, const double *u, long long dims, double z,
         double *f, double *J, double *T, double *rates);
+/* BE_chem_solve(calculate_rhs_primordial, calculate_jacobian_primordial, ...) as the Cython
+   harness calls it (dengo/solvers/BE_chem_solve.C:79-241,
+   rates_and_rate_tables_run.pyx.template:142) on host arrays: one backward-Euler
+   step of nstrip cells that share dt and the Newton convergence decision; u and
+   atol are scaled and unscaled in place as the reference does; the redshift is
+   data->current_z.  Returns 0 (converged), 1 (not converged in 10 sweeps, or a
+   failed RHS/Jacobian/NaN update), or a negative error code. */
+int dengo_primordial_be_chem_solve(primordial_data *data, double *u, double dt, double *rtol,
+        double *atol, int nstrip, int nchem, double *scaling);
+/* ensure_electron_consistency (rates_and_rate_tables.C.template:604-630): de of
+   cell i becomes the free-electron total of cells 0..i (the reference's total
+   runs across the strip). */
+void ensure_electron_consistency_primordial(double *input, int nstrip, int nchem);
+/* Sets data->current_z, the redshift the strip solve uses. */
+void dengo_primordial_set_current_z(primordial_data *data, double z);
+/* Copies data->Ts (temperatures of the last strip RHS evaluation) for
+   min(nstrip, MAX_NCELLS) cells; returns that count. */
+int dengo_primordial_strip_temperatures(const primordial_data *data, double *T, int nstrip);
 #ifdef __cplusplus
 }
 static inline double dengo_evolve_primordial(double dtf, double &dt, double z, double *input,

@@ -656,78 +656,167 @@ int orc_gauss_elim(double *A, double *x, double *b, int n)
     return 0;
 }
 
-/* BE_chem_solve.C:246-275: g(u) = u - u0 - dt f(u) in scaled variables. */
-static int be_resid(orc_rhs_f f, double *u, double *u0, double *gu, double dt, int n,
-                    double *sc, double *inv, void *sdata)
+/* BE_chem_solve.C:246-275: g(u) = u - u0 - dt f(u) in scaled variables, for
+ * the m = nstrip * nchem entries of a strip. */
+static int be_resid(orc_rhs_f f, double *u, double *u0, double *gu, double dt, int nstrip,
+                    int n, double *sc, double *inv, void *sdata)
 {
-    for (int i = 0; i < n; ++i) u[i] *= sc[i];
-    if (f(u, gu, 1, n, sdata) != 0) return 1;
-    for (int i = 0; i < n; ++i) u[i] *= inv[i];
-    for (int i = 0; i < n; ++i) gu[i] *= inv[i];
-    for (int i = 0; i < n; ++i) gu[i] = u[i] - u0[i] - dt * gu[i];
+    const int m = nstrip * n;
+    for (int i = 0; i < m; ++i) u[i] *= sc[i];
+    if (f(u, gu, nstrip, n, sdata) != 0) return 1;
+    for (int i = 0; i < m; ++i) u[i] *= inv[i];
+    for (int i = 0; i < m; ++i) gu[i] *= inv[i];
+    for (int i = 0; i < m; ++i) gu[i] = u[i] - u0[i] - dt * gu[i];
     return 0;
 }
 
-/* BE_chem_solve.C:280-321: J = I - dt * diag(inv) Jf diag(sc). */
-static int be_jac(orc_jac_f J, double *u, double *Ju, double dt, int n,
+/* BE_chem_solve.C:280-321: J = I - dt * diag(inv) Jf diag(sc), per cell of
+ * the strip (Ju[cell][col][row]). */
+static int be_jac(orc_jac_f J, double *u, double *Ju, double dt, int nstrip, int n,
                   double *sc, double *inv, void *sdata)
 {
-    for (int i = 0; i < n; ++i) u[i] *= sc[i];
-    if (J(u, Ju, 1, n, sdata) != 0) return 1;
-    for (int i = 0; i < n; ++i) u[i] *= inv[i];
-    for (int c = 0; c < n; ++c)
-        for (int r = 0; r < n; ++r) Ju[c * n + r] *= inv[r];
-    for (int r = 0; r < n; ++r)
-        for (int c = 0; c < n; ++c) Ju[c * n + r] *= sc[c];
-    for (int k = 0; k < n * n; ++k) Ju[k] = -dt * Ju[k];
-    for (int r = 0; r < n; ++r) Ju[r * n + r] += 1.0;
+    const int m = nstrip * n;
+    for (int i = 0; i < m; ++i) u[i] *= sc[i];
+    if (J(u, Ju, nstrip, n, sdata) != 0) return 1;
+    for (int i = 0; i < m; ++i) u[i] *= inv[i];
+    for (int x = 0; x < nstrip; ++x)
+        for (int c = 0; c < n; ++c)
+            for (int r = 0; r < n; ++r) Ju[(x * n + c) * n + r] *= inv[x * n + r];
+    for (int x = 0; x < nstrip; ++x)
+        for (int r = 0; r < n; ++r)
+            for (int c = 0; c < n; ++c) Ju[(x * n + c) * n + r] *= sc[x * n + c];
+    for (int k = 0; k < m * n; ++k) Ju[k] = -dt * Ju[k];
+    for (int x = 0; x < nstrip; ++x)
+        for (int r = 0; r < n; ++r) Ju[x * n * n + r * n + r] += 1.0;
     return 0;
 }
 
-/* BE_chem_solve.C:79-241 for one cell: <= 10 sweeps, lambda = 1, max-norm
- * test in scaled units that stops at the first unconverged entry, NaN exit. */
+/* BE_chem_solve.C:79-241: <= 10 sweeps, lambda = 1; per cell of the strip a
+ * Gauss_Elim solve, the update, and a max-norm test in scaled units that
+ * stops at the cell's first unconverged entry; a NaN met before it ends the
+ * call (the remaining cells keep their state); the strip is converged when
+ * every cell is. */
 int orc_BE_chem_solve(orc_rhs_f f, orc_jac_f J, double *u, double dt, double *rtol,
                       double *atol, int nstrip, int n, double *sc, void *sdata,
                       double *u0, double *s, double *gu, double *Ju)
 {
+    const int m = nstrip * n;
     double inv_small[64];
-    double *inv = n <= 64 ? inv_small : (double *)malloc(sizeof(double) * n);
+    double *inv = m <= 64 ? inv_small : (double *)malloc(sizeof(double) * m);
     int unsolved = 0, rv = 0;
-    (void)nstrip;
-    for (int i = 0; i < n; ++i) inv[i] = 1.0 / sc[i];
-    for (int i = 0; i < n; ++i) u[i] *= inv[i];
-    for (int i = 0; i < n; ++i) atol[i] *= inv[i];
-    for (int i = 0; i < n; ++i) u0[i] = u[i];
+    for (int i = 0; i < m; ++i) inv[i] = 1.0 / sc[i];
+    for (int i = 0; i < m; ++i) u[i] *= inv[i];
+    for (int i = 0; i < m; ++i) atol[i] *= inv[i];
+    for (int i = 0; i < m; ++i) u0[i] = u[i];
     for (int i = 0; i < n; ++i) s[i] = 0.0;
-    for (int i = 0; i < n; ++i) gu[i] = 0.0;
-    for (int i = 0; i < n * n; ++i) Ju[i] = 0.0;
+    for (int i = 0; i < m; ++i) gu[i] = 0.0;
+    for (int i = 0; i < m * n; ++i) Ju[i] = 0.0;
     for (int sweep = 0; sweep < 10; ++sweep) {
-        if (be_resid(f, u, u0, gu, dt, n, sc, inv, sdata) != 0 ||
-            be_jac(J, u, Ju, dt, n, sc, inv, sdata) != 0) {
+        if (be_resid(f, u, u0, gu, dt, nstrip, n, sc, inv, sdata) != 0 ||
+            be_jac(J, u, Ju, dt, nstrip, n, sc, inv, sdata) != 0) {
             rv = 1;
             goto done;
         }
         unsolved = 0;
-        orc_gauss_elim(Ju, s, gu, n);
-        for (int i = 0; i < n; ++i) u[i] -= 1.0 * s[i];
-        for (int i = 0; i < n; ++i) {
-            if (fabs(s[i]) > (atol[i] + rtol[i] * fabs(u[i]))) {
-                unsolved = 1;
-                break;
-            }
-            if (u[i] != u[i]) {
-                rv = 1;
-                goto done;
+        for (int x = 0; x < nstrip; ++x) {
+            const int o = x * n;
+            orc_gauss_elim(Ju + (size_t)x * n * n, s, gu + o, n);
+            for (int i = 0; i < n; ++i) u[o + i] -= 1.0 * s[i];
+            for (int i = 0; i < n; ++i) {
+                if (fabs(s[i]) > (atol[o + i] + rtol[o + i] * fabs(u[o + i]))) {
+                    unsolved = 1;
+                    break;
+                }
+                if (u[o + i] != u[o + i]) {
+                    rv = 1;
+                    goto done;
+                }
             }
         }
         if (!unsolved) break;
     }
     rv = unsolved ? 1 : 0;
 done:
-    for (int i = 0; i < n; ++i) u[i] *= sc[i];
-    for (int i = 0; i < n; ++i) atol[i] *= sc[i];
+    for (int i = 0; i < m; ++i) u[i] *= sc[i];
+    for (int i = 0; i < m; ++i) atol[i] *= sc[i];
     if (inv != inv_small) free(inv);
     return rv;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Strips of several cells: the Cython harness's call of BE_chem_solve       */
+/* (rates_and_rate_tables_run.pyx.template:140-178) on one strip of every    */
+/* cell.  The strip RHS / Jacobian evaluate each cell with its own per-cell  */
+/* state (temperature and interpolated rates of that cell's last RHS) and    */
+/* fail when any cell's does (calculate_rhs / calculate_jacobian return 1    */
+/* for a negative species, rates_and_rate_tables.C.template:500-505, 567).   */
+/* ------------------------------------------------------------------------ */
+static int strip_rhs(double *u, double *f, int nstrip, int nchem, void *sdata)
+{
+    orc_prim_data *d = (orc_prim_data *)sdata;
+    int rc = 0;
+    for (int x = 0; x < nstrip; ++x)
+        if (any_rhs(u + (size_t)x * nchem, f + (size_t)x * nchem, 1, nchem, d + x) != 0) rc = 1;
+    return rc;
+}
+
+static int strip_jac(double *u, double *J, int nstrip, int nchem, void *sdata)
+{
+    orc_prim_data *d = (orc_prim_data *)sdata;
+    int rc = 0;
+    for (int x = 0; x < nstrip; ++x)
+        if (any_jac(u + (size_t)x * nchem, J + (size_t)x * nchem * nchem, 1, nchem, d + x) != 0)
+            rc = 1;
+    return rc;
+}
+
+int orc_strip_be(const orc_ma_net *net, const double *tables, int nT, double T0, double T1,
+                 double z, int flags, double *u, double dt, double *rtol, double *atol,
+                 int nstrip, double *scaling, double *T_out, orc_be_f be)
+{
+    const int n = net ? net->n : ORC_PRIM_N;
+    if (be == NULL) be = orc_BE_chem_solve;
+    orc_prim_data *cells = (orc_prim_data *)calloc((size_t)nstrip, sizeof(orc_prim_data));
+    double *u0 = (double *)malloc(sizeof(double) * (size_t)nstrip * n);
+    double *s = (double *)malloc(sizeof(double) * (size_t)n);
+    double *gu = (double *)malloc(sizeof(double) * (size_t)nstrip * n);
+    double *Ju = (double *)malloc(sizeof(double) * (size_t)nstrip * n * n);
+    for (int x = 0; x < nstrip; ++x) {
+        orc_prim_setup(&cells[x], tables, nT, T0, T1);
+        cells[x].flags = flags;
+        cells[x].current_z = z;
+        if (net) {
+            cells[x].net = net;
+            cells[x].n = net->n;
+            cells[x].ntab = net->ntab;
+        }
+    }
+    const int rv = be(strip_rhs, strip_jac, u, dt, rtol, atol, nstrip, n, scaling, cells, u0, s,
+                      gu, Ju);
+    if (T_out)
+        for (int x = 0; x < nstrip; ++x) T_out[x] = cells[x].T;
+    free(cells);
+    free(u0);
+    free(s);
+    free(gu);
+    free(Ju);
+    return rv;
+}
+
+/* ensure_electron_consistency (rates_and_rate_tables.C.template:604-630) on
+ * a strip: the free-electron total runs across the cells. */
+void orc_strip_electrons(const orc_ma_net *net, double *input, int nstrip)
+{
+    const int n = net ? net->n : ORC_PRIM_N;
+    const int ge = n - 1, de = net ? net->de : DE;
+    const double *Fe = net ? net->free_e : FREE_E;
+    double total_e = 0.0;
+    for (int x = 0; x < nstrip; ++x) {
+        double *c = input + (size_t)x * n;
+        for (int k = 0; k < n; ++k)
+            if (k != ge && k != de) total_e += c[k] * Fe[k];
+        c[de] = total_e;
+    }
 }
 
 /* ------------------------------------------------------------------------ */

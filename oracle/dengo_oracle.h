@@ -137,6 +137,17 @@ long long orc_big_eval(const orc_big_net *net, const double *Tcell, const double
                        long long dims, double *f, double *J);
 
 /* Mass-action network counterparts of orc_prim_evolve / orc_prim_eval. */
+/* One BE_chem_solve (be; NULL selects orc_BE_chem_solve) of a strip of
+ * nstrip cells of the primordial network (net NULL) or a mass-action
+ * network, as the Cython harness calls it: u, atol [nstrip][n] in place,
+ * scaling [nstrip][n]; T_out[nstrip] receives each cell's temperature of the
+ * last RHS evaluation.  Returns the solver's 0 / 1. */
+int orc_strip_be(const orc_ma_net *net, const double *tables, int nT, double T0, double T1,
+                 double z, int flags, double *u, double dt, double *rtol, double *atol,
+                 int nstrip, double *scaling, double *T_out, orc_be_f be);
+/* ensure_electron_consistency on a strip (the total runs across cells). */
+void orc_strip_electrons(const orc_ma_net *net, double *input, int nstrip);
+
 double orc_ma_evolve(const orc_ma_net *net, double dtf, double *dt, double z, double *input,
                      double *rtol, double *atol, long long dims, const double *tables,
                      int nT, double T0, double T1, int input_is_number, int *status,
