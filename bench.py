@@ -38,6 +38,11 @@ sys.path.insert(0, ROOT)
 # (45 multipliers, 570 update, 90 forward, 100 back-substitution), update and
 # convergence test 30.
 F_SWEEP = 2137
+# Algorithmic FP64 operations per Newton sweep of the dense 128-species solve
+# (the reference's algorithm, whatever elimination the kernel uses):
+# Gauss_Elim sum_{m=1}^{127} (2 m^2 + 3 m) + 128^2 = 1,422,528, Jacobian
+# assembly and scaling about 80,000, RHS about 10,000.
+F_SWEEP_DENSE = 1512528
 PEAK_FP64 = 78.6e12      # MI355X FP64 (vector and matrix) TFLOP/s
 PEAK_FP32 = 157.3e12     # MI355X FP32 vector
 
@@ -87,6 +92,56 @@ def serving_leg(args, prim):
     return out
 
 
+def parity_stats(x, st, nsub, nrej, x_ref, st_ref, nsub_ref, nrej_ref):
+    """Status agreement, same-substep-path fraction and per-cell max relative
+    error (cells both runs take to dtf) of a run against a reference run."""
+    import torch
+    done = (st_ref == 0) & (st == 0)
+    rel = ((x - x_ref).abs() / x_ref.abs().clamp_min(1e-300))[done].max(dim=1).values
+    same = (st == st_ref) & (nsub == nsub_ref) & (nrej == nrej_ref)
+    return {"status_agreement": float((st == st_ref).float().mean()),
+            "same_path": float(same.float().mean()),
+            "cells_within_1e-6": float((rel <= 1e-6).float().mean()) if rel.numel() else 1.0,
+            "bitwise_equal": bool(torch.equal(x, x_ref))}
+
+
+def network_leg(name, solver, x0, rtol, atol, variants, dtf=1e12, T=None, flops_per_sweep=None):
+    """Runs every variant once on device copies of the same inputs; the first
+    variant is the parity reference.  Returns per-variant kernel time,
+    cell-solves/s and parity statistics."""
+    import torch
+    from dengo_amd import runtime
+    dev = torch.device("cuda:0")
+    n = x0.shape[0]
+    d_rt = torch.from_numpy(rtol).to(dev)
+    out, ref = {}, None
+    for label, prec in variants:
+        x = torch.from_numpy(x0).to(dev)
+        at = torch.from_numpy(atol).to(dev)
+        o = {k: torch.zeros(n, dtype=torch.int32, device=dev)
+             for k in ("status", "nsub", "nrej", "nsweep")}
+        if T is None:
+            solver.evolve_device(x, d_rt, at, dtf, outputs=o, precision=prec)
+        else:
+            solver.evolve_device(x, torch.from_numpy(T).to(dev), d_rt, at, dtf, outputs=o,
+                                 precision=prec)
+        ms = runtime.load().dengo_last_kernel_ms()
+        rec = {"kernel_ms": ms, "cell_solves_per_s": n / (ms * 1e-3),
+               "status_counts": torch.bincount(o["status"].long(), minlength=3).tolist()}
+        if ref is None:
+            ref = (x, o)
+            sweeps = int(o["nsweep"].long().sum().item())
+            rec["sweeps"] = sweeps
+            if flops_per_sweep:
+                rec["fp64_tflops_algorithmic"] = sweeps * flops_per_sweep / (ms * 1e-3) / 1e12
+        else:
+            rec.update(parity_stats(x, o["status"], o["nsub"], o["nrej"], ref[0],
+                                    ref[1]["status"], ref[1]["nsub"], ref[1]["nrej"]))
+        out[label] = rec
+        log("%s %s: %.1f ms" % (name, label, ms))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +157,10 @@ def main():
     ap.add_argument("--no-percell", action="store_true", help="skip the plain per-cell kernel")
     ap.add_argument("--cpu-cells", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--oxygen-cells", type=int, default=1 << 20,
+                    help="oxygen ion-by-ion leg (0 skips it)")
+    ap.add_argument("--synthetic-cells", type=int, default=65536,
+                    help="dense 128-species leg (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=2.0,
                     help="patch server: Poisson arrival rate (patches/s)")
     ap.add_argument("--serve-seconds", type=float, default=15.0,
@@ -239,6 +298,35 @@ def main():
             "traffic": None, "flops_per_launch": flops, "F_sweep": F_SWEEP,
             "sweeps_per_launch": sweeps}
 
+    workloads = {}
+    if args.oxygen_cells > 0:
+        from dengo_amd.ics import oxygen_ics
+        oxy = runtime.OxygenSolver()
+        xo, _, _ = oxygen_ics(args.oxygen_cells, seed=args.seed + 1)
+        rto, ato = tolerances(xo)
+        workloads["oxygen"] = {
+            "workload": "O I-IX ion-by-ion, 11 unknowns, %d cells, dtf=1e12 s" % args.oxygen_cells,
+            "variants": network_leg("oxygen", oxy, xo, rto, ato,
+                                    [("fp64_persistent", runtime.FP64),
+                                     ("fp64fast_persistent", runtime.FP64_FAST),
+                                     ("mixed_persistent", runtime.MIXED)])}
+        oxy.close()
+        del xo, rto, ato
+    if args.synthetic_cells > 0:
+        from dengo_amd.synthetic import SyntheticNetwork, synthetic_ics
+        net = SyntheticNetwork(seed=7)
+        syn = runtime.SyntheticSolver(net)
+        xs, Ts = synthetic_ics(net, args.synthetic_cells, seed=args.seed + 2)
+        rts, ats = tolerances(xs)
+        workloads["synthetic"] = {
+            "workload": "synthetic 128 species / 1500 reactions, dense Jacobian, %d cells, "
+                        "dtf=1e12 s" % args.synthetic_cells,
+            "variants": network_leg("synthetic", syn, xs, rts, ats,
+                                    [("fp64fast_persistent", runtime.FP64_FAST)], T=Ts,
+                                    flops_per_sweep=F_SWEEP_DENSE)}
+        syn.close()
+        del xs, Ts, rts, ats
+
     serve = None
     if args.serve_seconds > 0:
         serve = serving_leg(args, solver)
@@ -275,6 +363,7 @@ def main():
         "other_variants": extra,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "other_workloads": workloads,
         "serving": serve,
     }
     print(json.dumps(out))

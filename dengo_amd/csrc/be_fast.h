@@ -12,9 +12,9 @@
 // * the reference's row/column scaling (1/scale, scale) is folded into the
 //   Jacobian assembly;
 // * LU precision is a template parameter: double, or float on a two-sided
-//   equilibrated copy with iterative refinement against the FP64 block (FP64
-//   residual and FP64 state; the Newton loop re-evaluates the FP64 nonlinear
-//   residual every sweep).
+//   equilibrated copy with iterative refinement against the FP64 block and an
+//   FP64 fallback (FP64 residual and FP64 state; the Newton loop re-evaluates
+//   the FP64 nonlinear residual every sweep).
 #pragma once
 #include "be_driver.h"
 
@@ -315,24 +315,18 @@ DENGO_D void begin_be_fast(Cell<Net> &c)
     c.sweep = 0;
 }
 
-// Iterative refinement of the FP32 solve: at most kRefineSteps steps (each one
-// FP64 residual of the block and one FP32 triangular solve), stopped once a
-// correction is below kRefineTol of the Newton test's tolerance in every
-// component; a solve that has not got there is redone by the FP64 LU of the
-// same block, so an FP32 factor that cannot resolve the step (condition
-// number near 1/eps_fp32) never decides the Newton iteration.
-#ifndef DENGO_REFINE_STEPS
-#define DENGO_REFINE_STEPS 3
-#endif
-#ifndef DENGO_REFINE_TOL
-#define DENGO_REFINE_TOL 1e-3
-#endif
-#ifndef DENGO_REFINE_MODE
-#define DENGO_REFINE_MODE 1
-#endif
-constexpr int kRefineSteps = DENGO_REFINE_STEPS;
-constexpr int kRefineMode = DENGO_REFINE_MODE;
-constexpr double kRefineTol = DENGO_REFINE_TOL;
+// Iterative refinement of the FP32 solve (mixed precision): at most
+// kRefineSteps steps, each one FP64 residual of the block and one FP32
+// triangular solve.  A solve is accepted once the last correction of every
+// unknown -- the species and, through its row, the energy -- is below
+// kRefineTol of both the Newton test's tolerance and the unknown's updated
+// scaled value, and no unknown would become negative.  Otherwise the step is
+// redone by the FP64 LU of the same block, so the FP32 factor never decides
+// an update it cannot resolve: the Newton iteration sees an FP64-accurate
+// correction, componentwise, including components many decades below the
+// largest.
+constexpr int kRefineSteps = 4;
+constexpr double kRefineTol = 1e-3;
 
 template <class Net, class LUT>
 DENGO_D int sweep_fast(Cell<Net> &c, const KParams<Net> &p)
@@ -408,6 +402,7 @@ DENGO_D int sweep_fast(Cell<Net> &c, const KParams<Net> &p)
 #pragma unroll
         for (int i = 0; i < M; ++i)
             tol[i] = kRefineTol * (c.atol[i] + c.rtol[i] * fabs(c.u[i] * c.inv[i]));
+        const double tol_ge = kRefineTol * (c.atol[GE] + c.rtol[GE] * fabs(c.u[GE] * c.inv[GE]));
         bool refined = false;
         for (int it = 0; it < kRefineSteps && !refined; ++it) {
             static_for<0, M>([&](auto ii) {
@@ -421,25 +416,29 @@ DENGO_D int sweep_fast(Cell<Net> &c, const KParams<Net> &p)
             });
             block_lu_apply<M, FT::plan>(A32, rp, piv, y32);
             bool small = true;
+            double dge = 0.0, yg = g[GE];
 #pragma unroll
             for (int i = 0; i < M; ++i) {
                 const double dy = e[i] * (double)y32[i];
                 y[i] = y[i] + dy;
-                if constexpr (kRefineMode == 2) {
-                    const double un = fabs(c.u[i] * c.inv[i] - y[i]);
-                    small = small && fabs(dy) <= fmin(tol[i], kRefineTol * un);
-                } else {
-                    small = small && fabs(dy) <= tol[i];
-                }
+                dge = dge + age[i] * dy;
+                yg = yg - age[i] * y[i];
+                const double un = fabs(c.u[i] * c.inv[i] - y[i]);
+                small = small && fabs(dy) <= fmin(tol[i], kRefineTol * un);
             }
+            const double un = fabs(c.u[GE] * c.inv[GE] - yg);
+            small = small && fabs(dge) <= fmin(tol_ge, kRefineTol * un);
             refined = small;
         }
-        if constexpr (kRefineMode >= 1) {
-            // a component the FP32 step would drive negative is decided by FP64
+        if (refined) {
             bool neg = false;
+            double yg = g[GE];
 #pragma unroll
-            for (int i = 0; i < M; ++i) neg = neg || (c.u[i] - c.sc[i] * y[i] < 0.0);
-            refined = refined && !neg;
+            for (int i = 0; i < M; ++i) {
+                neg = neg || (c.u[i] - c.sc[i] * y[i] < 0.0);
+                yg = yg - age[i] * y[i];
+            }
+            refined = !(neg || (c.u[GE] - c.sc[GE] * yg < 0.0));
         }
         if (!refined) {
             double rp64[M];

@@ -302,14 +302,14 @@ __device__ __forceinline__ void factor_panel(double (&M)[TCW][RPL][TILE], Shared
         pos[h] = S.pos[G.row(h)];
         xr[h] = S.x[G.row(h)];
     }
-#pragma unroll 1
-    for (int kk = 0; kk < TILE; ++kk) {
+    // the 16 steps are unrolled so that the pivot column, the columns right of
+    // it and the LDS slots are compile-time register and address choices
+    static_for<0, TILE>([&](auto KKc) {
+        constexpr int kk = CV(KKc);
         const int k = TILE * P + kk;
         double v[RPL];
-        dispatch<0, TILE>(kk, [&](auto KK) {
 #pragma unroll
-            for (int h = 0; h < RPL; ++h) v[h] = M[TC][h][KK.value];
-        });
+        for (int h = 0; h < RPL; ++h) v[h] = M[TC][h][kk];
         // pivot search: largest |a| among rows not yet pivoted, first in the
         // swapped order on ties; a NaN wins only at position k (where the
         // reference's scan starts), elsewhere it never compares larger.
@@ -407,7 +407,7 @@ __device__ __forceinline__ void factor_panel(double (&M)[TCW][RPL][TILE], Shared
             S.prow[k] = p;
             S.pv[k] = piv;
         }
-    }
+    });
 #pragma unroll
     for (int h = 0; h < RPL; ++h) {
         S.pos[G.row(h)] = pos[h];

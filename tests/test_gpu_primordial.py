@@ -182,13 +182,16 @@ def test_fp64_fast_vs_oracle(solver, schedule):
 
 @pytest.mark.parametrize("schedule", [runtime.PER_CELL, runtime.PERSISTENT])
 def test_mixed_vs_oracle(solver, schedule):
-    """FP32-LU / FP64-residual variant.  Cells that reach dtf in both agree to
-    1e-6 relative on the same substep path.  Known limitation: in cells that
-    cool to the 1 K floor with species near the 1e-25 floor the FP32 exponent
-    range cannot carry the tiny update components, converged states turn
-    slightly negative and the step is rejected, so cells the reference runs to
-    the 50000-substep cap (status 2) give up instead (status 1).  Every status
-    disagreement must be of that kind."""
+    """FP32 LU with FP64 iterative refinement and an FP64 fallback.  Cells that
+    reach dtf in both agree to 1e-6 relative on the same substep path.  Known
+    difference: the refined solve is accurate componentwise, while in the
+    cells that cool to the 1 K floor with species near the 1e-25 floor the
+    reference's own FP64 LU error (relative to the largest component) drives
+    small species negative and so decides which substeps are rejected; the
+    refined solve does not reproduce those rejections, its trajectory
+    diverges, and cells the reference runs to the 50000-substep cap (status
+    2) give up instead (status 1).  Every status disagreement must be of that
+    kind."""
     g = np.load(GOLD)
     r = solver.evolve(g["input"], g["rtol"], g["atol"], float(g["dtf"]),
                       precision=runtime.MIXED, schedule=schedule)
