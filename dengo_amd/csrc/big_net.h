@@ -302,14 +302,25 @@ __device__ __forceinline__ void factor_panel(double (&M)[TCW][RPL][TILE], Shared
         pos[h] = S.pos[G.row(h)];
         xr[h] = S.x[G.row(h)];
     }
-    // the 16 steps are unrolled so that the pivot column, the columns right of
-    // it and the LDS slots are compile-time register and address choices
-    static_for<0, TILE>([&](auto KKc) {
-        constexpr int kk = CV(KKc);
+    // One pivot step; kkv is the step as a run-time int or as a compile-time
+    // constant.  The Gauss-Jordan mode unrolls the 16 steps, so that the pivot
+    // column, the columns right of it and the LDS slots become compile-time
+    // register and address choices (1.6x faster factorization); the faithful
+    // mode keeps the run-time loop, whose register allocation its back
+    // substitution buffers need.
+    auto step = [&](auto kkv) {
+        const int kk = (int)kkv;
         const int k = TILE * P + kk;
         double v[RPL];
+        if constexpr (std::is_same<decltype(kkv), int>::value) {
+            dispatch<0, TILE>(kk, [&](auto KK) {
 #pragma unroll
-        for (int h = 0; h < RPL; ++h) v[h] = M[TC][h][kk];
+                for (int h = 0; h < RPL; ++h) v[h] = M[TC][h][KK.value];
+            });
+        } else {
+#pragma unroll
+            for (int h = 0; h < RPL; ++h) v[h] = M[TC][h][decltype(kkv)::value];
+        }
         // pivot search: largest |a| among rows not yet pivoted, first in the
         // swapped order on ties; a NaN wins only at position k (where the
         // reference's scan starts), elsewhere it never compares larger.
@@ -352,9 +363,12 @@ __device__ __forceinline__ void factor_panel(double (&M)[TCW][RPL][TILE], Shared
         const int hp = b0 ? 0 : 1;
         const int lp = __builtin_amdgcn_readfirstlane(__builtin_ctzll(b0 ? b0 : b1));
         const int p = lp + 64 * hp;
+        // the pivot row's values: each lane selects its slot hp (a data
+        // select, wave-uniform), then one readlane broadcasts lane lp's value;
+        // selecting between two readlanes instead would be a branch per value
         const int posp = __builtin_amdgcn_readlane(hp ? pos[1] : pos[0], lp);
-        const double piv = hp ? readlane_d(v[1], lp) : readlane_d(v[0], lp);
-        const double xp = hp ? readlane_d(xr[1], lp) : readlane_d(xr[0], lp);
+        const double piv = readlane_d(hp ? v[1] : v[0], lp);
+        const double xp = readlane_d(hp ? xr[1] : xr[0], lp);
         // multipliers of this wave's rows (published for the other waves) and
         // the right-hand side
         double rp = 0.0;
@@ -364,17 +378,19 @@ __device__ __forceinline__ void factor_panel(double (&M)[TCW][RPL][TILE], Shared
         for (int h = 0; h < RPL; ++h) {
             const int row = G.row(h);
             bool inc = row != p;
-            if constexpr (MODE == kLU) inc = inc && cand[h];
-            l[h] = 0.0;
-            if (inc) {
-                if constexpr (MODE == kLU) {
+            if constexpr (MODE == kLU) {
+                inc = inc && cand[h];
+                l[h] = 0.0;
+                if (inc) {
                     l[h] = v[h] / piv;
                     const double t = l[h] * xp;
                     xr[h] = xr[h] - t;
-                } else {
-                    l[h] = v[h] * rp;
-                    xr[h] = fma(-l[h], xp, xr[h]);
                 }
+            } else {
+                const double lm = v[h] * rp;
+                const double xn = fma(-lm, xp, xr[h]);
+                l[h] = inc ? lm : 0.0;
+                xr[h] = inc ? xn : xr[h];
             }
             S.buf[kk * N + row] = l[h];
         }
@@ -384,7 +400,7 @@ __device__ __forceinline__ void factor_panel(double (&M)[TCW][RPL][TILE], Shared
 #pragma unroll
         for (int c = 0; c < TILE; ++c) {
             u[c] = 0.0;
-            if (c > kk) u[c] = hp ? readlane_d(M[TC][1][c], lq) : readlane_d(M[TC][0][c], lq);
+            if (c > kk) u[c] = readlane_d(hp ? M[TC][1][c] : M[TC][0][c], lq);
         }
         if constexpr (MODE == kLU) {
             if (G.lane == 0) {
@@ -393,10 +409,14 @@ __device__ __forceinline__ void factor_panel(double (&M)[TCW][RPL][TILE], Shared
                     if (c > kk) Ul[urow_off(k) + (TILE * P + c) - k - 1] = u[c];
             }
         }
+        // Gauss-Jordan never reads a column again once it has been the pivot
+        // column (its multipliers live in S.buf), so only columns right of kk
+        // are updated; the faithful LU keeps the full-width update
 #pragma unroll
         for (int h = 0; h < RPL; ++h)
 #pragma unroll
-            for (int c = 0; c < TILE; ++c) row_update<MODE>(M[TC][h][c], l[h], u[c]);
+            for (int c = 0; c < TILE; ++c)
+                if (MODE == kLU || c > kk) row_update<MODE>(M[TC][h][c], l[h], u[c]);
         // swapped-order positions: the row at position k moves to p's place
 #pragma unroll
         for (int h = 0; h < RPL; ++h) {
@@ -407,7 +427,13 @@ __device__ __forceinline__ void factor_panel(double (&M)[TCW][RPL][TILE], Shared
             S.prow[k] = p;
             S.pv[k] = piv;
         }
-    });
+    };
+    if constexpr (MODE == kGJ) {
+        static_for<0, TILE>(step);
+    } else {
+#pragma unroll 1
+        for (int kk = 0; kk < TILE; ++kk) step(kk);
+    }
 #pragma unroll
     for (int h = 0; h < RPL; ++h) {
         S.pos[G.row(h)] = pos[h];

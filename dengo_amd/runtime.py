@@ -146,6 +146,13 @@ SIGNATURES.update({
     "dengo_server_get_stats": (None, [ctypes.c_void_p, ctypes.POINTER(ServerStats)]),
     "dengo_server_now": (ctypes.c_double, [ctypes.c_void_p]),
     "dengo_server_destroy": (None, [ctypes.c_void_p]),
+    "dengo_graph_create": (ctypes.c_void_p, [ctypes.c_int, ctypes.c_void_p,
+                                             ctypes.POINTER(Options), ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_longlong, ctypes.POINTER(CellOutputs)]),
+    "dengo_graph_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "dengo_graph_destroy": (None, [ctypes.c_void_p]),
     "dengo_host_alloc": (ctypes.c_void_p, [ctypes.c_size_t]),
     "dengo_host_free": (None, [ctypes.c_void_p]),
 })
@@ -574,6 +581,55 @@ class PatchServer(object):
             load().dengo_server_destroy(self.srv)
             self.srv = None
             self._keep = {}
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class CapturedEvolve(object):
+    """One integration of device-resident torch tensors captured as a HIP
+    graph (include/dengo_graph.h); launch() replays it in place on a stream.
+    solver: PrimordialSolver, OxygenSolver or SyntheticSolver."""
+
+    def __init__(self, solver, x, rtol, atol, dtf, T=None, outputs=None, dt0=-1.0, z=-1.0,
+                 **opt):
+        if isinstance(solver, SyntheticSolver):
+            net = NET_SYNTHETIC
+        elif getattr(solver, "name", None) == "oxygen":
+            net = NET_OXYGEN
+        elif getattr(solver, "name", None) == "primordial":
+            net = NET_PRIMORDIAL
+        else:
+            raise ValueError("graph capture serves the built-in networks")
+        for t in (x, rtol, atol):
+            if not (t.is_cuda and t.element_size() == 8 and t.is_contiguous() and t.dim() == 2
+                    and t.shape[1] == solver.N):
+                raise ValueError("expected contiguous float64 cuda tensors [n, %d]" % solver.N)
+        co = CellOutputs()
+        for k, v in (outputs or {}).items():
+            setattr(co, k, v.data_ptr())
+        o = options(**opt)
+        o.block = 0
+        self._keep = (x, rtol, atol, T, outputs)
+        self.g = load().dengo_graph_create(
+            net, solver.data, ctypes.byref(o), dtf, dt0, z,
+            None if T is None else ctypes.c_void_p(T.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+            ctypes.c_void_p(rtol.data_ptr()), ctypes.c_void_p(atol.data_ptr()), x.shape[0],
+            ctypes.byref(co))
+        if not self.g:
+            raise RuntimeError("dengo_graph_create failed: %s" % last_error())
+
+    def launch(self, stream=None):
+        s = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+        _check(load().dengo_graph_launch(self.g, s), "dengo_graph_launch")
+
+    def close(self):
+        if getattr(self, "g", None):
+            load().dengo_graph_destroy(self.g)
+            self.g = None
 
     def __del__(self):
         try:
