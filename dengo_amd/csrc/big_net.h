@@ -93,6 +93,7 @@ struct NetDev {
 struct Shared {
     double k[MAX_RXN];
     double buf[BUF_SZ];                // fluxes; a Jacobian chunk; a panel's multipliers
+    double buf2[CHUNK_SZ];             // the other Jacobian chunk buffer
     double u[N], u0[N], sc[N], inv[N], prev[N], at[N], rt[N], x[N], s[N], pv[N];
     double f[N + 1];
     unsigned short ab[MAX_RXN];        // a | b << 8
@@ -253,6 +254,71 @@ __device__ __forceinline__ void run_jac_chunk(const NetDev &net, Shared &S, cons
         for (int i = 0; i < OPS_BATCH; ++i) cur[i] = nxt[i];
     }
     if (e_cur >= 0) S.buf[e_cur] = acc;
+}
+
+// The BE matrix M = I - dt diag(inv) Jf diag(sc) (BE_Resid_Jac, :280-321)
+// from the 8 Jacobian column chunks.  Chunk J is gathered into one of two
+// LDS buffers while the owner wave of chunk J-1 scales the other one into
+// its registers: one barrier per chunk.  Per entry the operations are those
+// of run_jac_chunk, in the same order.
+__device__ __forceinline__ void assemble_jacobian(const NetDev &net, Shared &S, const Geo &G,
+                                                  double (&M)[TCW][RPL][TILE], double mdt)
+{
+    const unsigned t = G.t;
+    static_for<0, NT>([&](auto JC) {
+        constexpr int J = CV(JC);
+        double *cb = (J & 1) ? S.buf2 : S.buf;
+        const unsigned *ops = net.jac_ops + (size_t)net.jac_off[J] * NTHR;
+        const int L = net.jac_len[J];
+        unsigned cur[OPS_BATCH], nxt[OPS_BATCH];
+        if (L > 0) load_ops(ops, t, 0, cur);
+        double acc = 0.0;
+        int e_cur = -1;
+        for (int q = 0; q < L; q += OPS_BATCH) {
+            if (q + OPS_BATCH < L) load_ops(ops, t, q + OPS_BATCH, nxt);
+#pragma unroll
+            for (int i = 0; i < OPS_BATCH; ++i) {
+                const unsigned o = cur[i];
+                const int e = (int)((o >> 7) & 15) * CHUNK_LD + (int)(o & 127);
+                const int nu = (int)((o >> 11) & 7) - 4, partner = (o >> 14) & 127, r = o >> 21;
+                if (e != e_cur) {
+                    if (e_cur >= 0) cb[e_cur] = acc;
+                    acc = 0.0;
+                    e_cur = e;
+                }
+                if (nu != 0) {
+                    const double part = S.k[r] * S.u[partner];
+                    acc = acc + (double)nu * part;
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < OPS_BATCH; ++i) cur[i] = nxt[i];
+        }
+        if (e_cur >= 0) cb[e_cur] = acc;
+        __syncthreads();
+        if (G.w == J % NWAVE) {
+            constexpr int TC = J / NWAVE;
+            double iv[RPL];
+#pragma unroll
+            for (int h = 0; h < RPL; ++h) iv[h] = S.inv[G.row(h)];
+#pragma unroll
+            for (int c = 0; c < TILE; ++c) {
+                const int colg = TILE * J + c;
+                const double scc = S.sc[colg];
+#pragma unroll
+                for (int h = 0; h < RPL; ++h) {
+                    const int row = G.row(h);
+                    double val = cb[c * CHUNK_LD + row];
+                    val = val * iv[h];
+                    val = val * scc;
+                    val = mdt * val;
+                    if (row == colg) val = val + 1.0;
+                    M[TC][h][c] = val;
+                }
+            }
+        }
+    });
+    __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
@@ -447,13 +513,24 @@ __device__ __forceinline__ void apply_panel(double (&M)[TCW][RPL][TILE], Shared 
                                             int P, double *Ul)
 {
     const int Jc = G.w + NWAVE * TC;
+    // the next step's pivot row index and multipliers are read from LDS while
+    // this step's updates run
+    int p_next = S.prow[TILE * P];
+    double l_next[RPL];
+#pragma unroll
+    for (int h = 0; h < RPL; ++h) l_next[h] = S.buf[G.row(h)];
 #pragma unroll 1
     for (int s = 0; s < TILE; ++s) {
         const int k = TILE * P + s;
-        const int p = __builtin_amdgcn_readfirstlane(S.prow[k]);
+        const int p = __builtin_amdgcn_readfirstlane(p_next);
         double l[RPL];
 #pragma unroll
-        for (int h = 0; h < RPL; ++h) l[h] = S.buf[s * N + G.row(h)];
+        for (int h = 0; h < RPL; ++h) l[h] = l_next[h];
+        if (s + 1 < TILE) {
+            p_next = S.prow[k + 1];
+#pragma unroll
+            for (int h = 0; h < RPL; ++h) l_next[h] = S.buf[(s + 1) * N + G.row(h)];
+        }
         double u[TILE];
         pivot_row<TC>(M, p, u);
         if constexpr (MODE == kLU) {
@@ -562,33 +639,7 @@ __device__ int be_solve(const NetDev &net, Shared &S, const Geo &G, double (&M)[
         __syncthreads();
         // BE_Resid_Jac (:280-321): J = I - dt diag(inv) Jf diag(sc)
         const double mdt = -dt;
-        static_for<0, NT>([&](auto JC) {
-            constexpr int J = CV(JC);
-            run_jac_chunk(net, S, G, J);
-            __syncthreads();
-            if (G.w == J % NWAVE) {
-                constexpr int TC = J / NWAVE;
-                double iv[RPL];
-#pragma unroll
-                for (int h = 0; h < RPL; ++h) iv[h] = S.inv[G.row(h)];
-#pragma unroll
-                for (int c = 0; c < TILE; ++c) {
-                    const int colg = TILE * J + c;
-                    const double scc = S.sc[colg];
-#pragma unroll
-                    for (int h = 0; h < RPL; ++h) {
-                        const int row = G.row(h);
-                        double val = S.buf[c * CHUNK_LD + row];
-                        val = val * iv[h];
-                        val = val * scc;
-                        val = mdt * val;
-                        if (row == colg) val = val + 1.0;
-                        M[TC][h][c] = val;
-                    }
-                }
-            }
-            __syncthreads();
-        });
+        assemble_jacobian(net, S, G, M, mdt);
         if (t < N) S.u[t] = S.u[t] * S.inv[t];
         dense_solve<MODE>(M, S, G, Ul);
         // update and convergence test (:170-190)
