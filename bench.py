@@ -44,6 +44,11 @@ F_SWEEP = 2137
 # assembly and scaling about 80,000, RHS about 10,000.
 F_SWEEP_DENSE = 1512528
 PEAK_FP64 = 78.6e12      # MI355X FP64 (vector and matrix) TFLOP/s
+# HBM bytes per launch of the FP64-fast persistent kernel on the 128^3
+# primordial workload: FETCH_SIZE + WRITE_SIZE from two rocprofv3 --pmc runs
+# of this script (--calib-copy-mib 1024), each scaled by the counter's
+# reading for the 1 GiB device copy: 1,059,131 + 606,576 KiB.
+HEADLINE_TRAFFIC_BYTES = 1665708 * 1024
 PEAK_FP32 = 157.3e12     # MI355X FP32 vector
 
 
@@ -69,8 +74,18 @@ def serving_leg(args, prim):
         p, kp = serving.make_patch(k, n, s, synthetic_net=net)
         patches.append(p)
         keep.append(kp)
+    # mixed precision where it holds the reference's own noise floor (the
+    # oxygen network); the structure-exploiting FP64 sweep for the primordial
+    # network, whose capped cells the mixed sweep does not reproduce, and for
+    # the dense network, which has no FP32 factorization
+    if args.serve_precision == "mixed":
+        policy = {runtime.NET_PRIMORDIAL: runtime.MIXED, runtime.NET_OXYGEN: runtime.MIXED,
+                  runtime.NET_SYNTHETIC: runtime.FP64_FAST}
+    else:
+        policy = {runtime.NET_PRIMORDIAL: runtime.FP64_FAST, runtime.NET_OXYGEN: runtime.MIXED,
+                  runtime.NET_SYNTHETIC: runtime.FP64_FAST}
     srv = runtime.PatchServer(prim, oxy, syn, precision=runtime.MIXED, slots=args.serve_slots,
-                              grid_share=args.serve_share)
+                              grid_share=args.serve_share, net_precision=policy)
     try:
         recs, t0 = serving.replay(srv, trace, patches, 1e12, timeout_s=900.0)
         stats = srv.stats()
@@ -79,7 +94,9 @@ def serving_leg(args, prim):
     out = serving.summarize(trace, recs, t0)
     out.update({"rate_per_s": args.serve_rate, "window_s": args.serve_seconds,
                 "slots": args.serve_slots, "grid_share": args.serve_share or args.serve_slots,
-                "precision": "mixed (primordial, oxygen); fp64-fast (synthetic dense)",
+                "precision": {serving.NET_NAMES[k]: {runtime.FP64: "fp64", runtime.MIXED: "mixed",
+                                                      runtime.FP64_FAST: "fp64-fast"}[v]
+                              for k, v in policy.items()},
                 "arena_peak_bytes": stats["arena_peak"], "arena_bytes": stats["arena_bytes"],
                 "latency": "submit (host arrays in pinned memory) to copy-back complete"})
     log("serving: p50 %.1f ms, p99 %.1f ms, %.0f cell-solves/s" % (
@@ -166,14 +183,17 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=15.0,
                     help="patch server: arrival window (s); 0 skips the serving leg")
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-precision", default="parity", choices=["parity", "mixed"],
+                    help="parity: mixed sweep where it holds the reference's noise floor "
+                         "(oxygen), FP64-fast elsewhere; mixed: mixed wherever it exists")
     ap.add_argument("--serve-share", type=int, default=0,
                     help="patch server: grid share (0: the number of slots)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU: this build has no multi-GPU path "
-              "(cells would be split by domain across GPUs; see DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on one GPU: there is no multi-GPU path "
+              "(cells would be split by domain across GPUs, one process per GPU)", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -295,7 +315,10 @@ def main():
     roof = {"bound": "mfma", "compute_unit": "FP64 vector ALU (MI355X FP64 vector and "
             "matrix peaks are both 78.6 TFLOP/s)", "achieved": achieved / 1e12,
             "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
-            "traffic": None, "flops_per_launch": flops, "F_sweep": F_SWEEP,
+            "traffic": (HEADLINE_TRAFFIC_BYTES if args.variant == "fp64fast_persistent"
+                        and n == 128 ** 3 else None),
+            "traffic_unit": "bytes per launch (calibrated PMC FETCH_SIZE + WRITE_SIZE)",
+            "flops_per_launch": flops, "F_sweep": F_SWEEP,
             "sweeps_per_launch": sweeps}
 
     workloads = {}

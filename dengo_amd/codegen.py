@@ -10,7 +10,8 @@ other FFIs can bind it; a C++ ``double&`` overload is kept for source
 compatibility).
 
 ``render_solver`` (called by ``ChemicalNetwork.write_solver``) writes that
-header plus a ctypes harness for a network whose kernels this build ships.
+header, the device descriptor, the kernel instantiation file and a ctypes
+harness; ``dengo_amd.build.build_solver`` compiles them.
 """
 import os
 

@@ -94,8 +94,9 @@ struct dengo_server {
 
     int net_precision(int net) const
     {
-        if (hooks[net].isothermal && precision == DENGO_MIXED) return DENGO_FP64_FAST;
-        return precision;
+        const int pr = cfg.precision[net] >= 0 ? cfg.precision[net] : precision;
+        if (hooks[net].isothermal && pr == DENGO_MIXED) return DENGO_FP64_FAST;
+        return pr;
     }
 
     size_t request_bytes(int net, long long n) const
@@ -300,6 +301,7 @@ void dengo_server_default_config(dengo_server_config *cfg)
     cfg->slots = 4;
     cfg->grid_share = 0;
     cfg->max_bypass = 8;
+    for (int k = 0; k < 3; ++k) cfg->precision[k] = -1;
 }
 
 dengo_server *dengo_server_create(const dengo_server_config *cfg_in, void *primordial,
@@ -322,6 +324,11 @@ dengo_server *dengo_server_create(const dengo_server_config *cfg_in, void *primo
         set_error("unknown precision %d", precision);
         return nullptr;
     }
+    for (int k = 0; k < kNets; ++k)
+        if (cfg.precision[k] > DENGO_FP64_FAST || cfg.precision[k] < -1) {
+            set_error("unknown precision %d for network %d", cfg.precision[k], k);
+            return nullptr;
+        }
     if (!primordial && !oxygen && !synthetic) {
         set_error("a server needs at least one network");
         return nullptr;

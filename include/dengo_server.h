@@ -50,7 +50,10 @@ typedef struct dengo_server_config {
                              of the grid that fills the GPU (0: slots) */
     int max_bypass;       /* times a waiting request may be passed by later
                              ones that fit (0: 8) */
-    int reserved[5];
+    int precision[3];     /* per network (enum dengo_network): the Newton
+                             sweep's precision, or -1 for the one given to
+                             dengo_server_create */
+    int reserved[2];
 } dengo_server_config;
 
 /* Completion record of one request. */
@@ -78,14 +81,16 @@ typedef struct dengo_server_stats {
     double first_submit, last_done;      /* server steady clock, s */
 } dengo_server_stats;
 
-/* Defaults: 8 GiB arena, 4 slots, grid_share = slots, max_bypass 8. */
+/* Defaults: 8 GiB arena, 4 slots, grid_share = slots, max_bypass 8, every
+ * network at the precision given to dengo_server_create. */
 void dengo_server_default_config(dengo_server_config *cfg);
 
 /* Creates a server on the current device for the given data structs (tables
  * set; NULL for a network it will not serve).  precision selects the Newton
- * sweep of each network's patches (enum dengo_precision; the dense synthetic
- * network supports DENGO_FP64 and DENGO_FP64_FAST, and DENGO_MIXED falls back
- * to DENGO_FP64_FAST there).  Returns NULL on failure (dengo_last_error()). */
+ * sweep of each network's patches unless cfg->precision overrides it for that
+ * network (enum dengo_precision; the dense synthetic network supports
+ * DENGO_FP64 and DENGO_FP64_FAST, and DENGO_MIXED falls back to
+ * DENGO_FP64_FAST there).  Returns NULL on failure (dengo_last_error()). */
 dengo_server *dengo_server_create(const dengo_server_config *cfg, void *primordial,
         void *oxygen, void *synthetic, int precision);
 

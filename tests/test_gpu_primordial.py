@@ -6,7 +6,7 @@ PORTABLE_LOG mode, so against that mode they must agree bit for bit on every
 cell, including cells that hit the 50000-substep cap (whose final state is
 chaotic: a 1-ulp input change moves it by tens of percent).  Against the
 libm-log oracle (the reference's arithmetic) cells that reach dtf must agree to
-the north-star tolerance of 1e-6 relative.
+1e-6 relative.
 """
 import ctypes
 import os
@@ -204,3 +204,16 @@ def test_mixed_vs_oracle(solver, schedule):
     same = ((r["nsub"] == g["libm_nsub"]) & (r["nrej"] == g["libm_nrej"]))[done]
     assert same.mean() >= 0.9
     assert rel[same].max() < 1e-6
+
+
+def test_config1_16cube_bitwise(solver, prim_tables):
+    """The 16^3 = 4096-cell primordial workload (seeded inputs, dtf = 1e12 s):
+    the faithful persistent kernel against the portable-log oracle, every
+    cell's state, atol, status and counters bit for bit."""
+    inp, _, _ = primordial_ics(16 ** 3, seed=2024)
+    rtol, atol = tolerances(inp)
+    r = solver.evolve(inp, rtol, atol, 1e12)
+    o = oracle.prim_evolve(inp, rtol, atol, prim_tables[1], 1e12, flags=oracle.PORTABLE_LOG)
+    for k in ("state", "atol", "status", "nsub", "nrej", "nsweep"):
+        assert np.array_equal(r[k], o[k]), k
+    assert r["ttot"] == o["ttot"]

@@ -121,6 +121,32 @@ def test_mix_equals_alone(nets, precision):
     assert any(iv[i + 1][0] < iv[i][1] for i in range(len(iv) - 1)), iv
 
 
+def test_per_network_precision(nets):
+    """cfg.precision overrides the server-wide precision network by network:
+    each patch equals the plain entry point at its own network's precision."""
+    plist = patches()[:4]
+    policy = {PRIM: runtime.FP64_FAST, OXY: runtime.MIXED}
+    prim, oxy, syn = nets
+    srv = runtime.PatchServer(prim, oxy, syn, precision=runtime.FP64, slots=4,
+                              net_precision=policy)
+    try:
+        work = []
+        for net, p in plist:
+            st, at = p["state"].copy(), p["atol"].copy()
+            status = np.full(st.shape[0], -1, np.int32)
+            work.append((net, p, srv.submit(net, st, p["rtol"], at, p["dtf"], status=status),
+                         st, at, status))
+        srv.drain(300.0)
+        recs = {r["ticket"]: r for r in srv.poll(1024)}
+    finally:
+        srv.close()
+    for net, p, t, st, at, status in work:
+        assert recs[t]["error"] == 0, recs[t]["message"]
+        ref = alone(nets, net, p, policy[net])
+        assert np.array_equal(st, ref[0]) and np.array_equal(at, ref[1])
+        assert np.array_equal(status, ref[2])
+
+
 def test_one_slot_server_equals_mix(nets):
     plist = patches()[:4]
     one, recs1, st1 = serve(nets, plist, runtime.MIXED, slots=1)

@@ -29,7 +29,7 @@ def test_server_config_defaults():
     cfg = runtime.ServerConfig()
     lib.dengo_server_default_config(cfg)
     assert cfg.hbm_budget == 8 << 30 and cfg.slots == 4 and cfg.max_bypass == 8
-    assert cfg.grid_share == 0
+    assert cfg.grid_share == 0 and list(cfg.precision) == [-1, -1, -1]
 
 
 def test_server_create_rejects_bad_arguments():
@@ -42,3 +42,6 @@ def test_server_create_rejects_bad_arguments():
     lib.dengo_server_default_config(cfg)
     assert not lib.dengo_server_create(cfg, None, None, None, runtime.MIXED)
     assert "at least one network" in runtime.last_error()
+    cfg.precision[1] = 7
+    assert not lib.dengo_server_create(cfg, None, None, None, runtime.MIXED)
+    assert "unknown precision 7 for network 1" in runtime.last_error()
