@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libdengo_hip.so")
 SOURCES = ["dengo_common.hip", "dengo_primordial.hip", "dengo_oxygen.hip", "dengo_synthetic.hip",
-           "dengo_server.hip", "dengo_graph.hip"]
+           "dengo_server.hip", "dengo_graph.hip", "dengo_h5.hip"]
 HEADERS = ["dengo_device.h", "be_driver.h", "be_fast.h", "ma_net.h", "dengo_solver.h", "net_oxygen.h",
            "big_net.h", "big_program.h", "arena.h"]
 
@@ -37,7 +37,7 @@ def build(verbose=False, extra=()):
     os.makedirs(LIBDIR, exist_ok=True)
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     cmd = [hipcc()] + HIPCC_FLAGS + list(extra) + [
-        "-I", CSRC, "-I", os.path.join(ROOT, "include"), "-o", LIB + ".tmp"] + srcs
+        "-I", CSRC, "-I", os.path.join(ROOT, "include"), "-o", LIB + ".tmp"] + srcs + ["-ldl"]
     if verbose:
         print(" ".join(cmd))
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -57,7 +57,8 @@ def build_solver(output_dir, solver_name, verbose=False):
         raise RuntimeError("%s not found: run write_solver first" % src)
     lib = os.path.join(output_dir, "lib%s_solver.so" % solver_name)
     cmd = [hipcc()] + HIPCC_FLAGS + ["-I", output_dir, "-I", CSRC, "-I", os.path.join(ROOT, "include"),
-                                     "-o", lib + ".tmp", src, os.path.join(CSRC, "dengo_common.hip")]
+                                     "-o", lib + ".tmp", src, os.path.join(CSRC, "dengo_common.hip"),
+                                     os.path.join(CSRC, "dengo_h5.hip"), "-ldl"]
     if verbose:
         print(" ".join(cmd))
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -80,7 +81,7 @@ def build_generated_examples():
     from dengo_amd.networks import ion_network
     out = os.path.join(GENERATED, "carbon")
     net = ion_network("C", seed=9)
-    net.write_solver("carbon", output_dir=out)
+    net.write_solver("carbon", output_dir=out, write_h5=False)
     return [build_solver(out, "carbon")]
 
 

@@ -257,16 +257,31 @@ class ChemicalNetwork(object):
             json.dump(manifest, f, indent=1)
         return blob
 
+    def write_tables_h5(self, solver_name, output_dir="."):
+        """Write ``<name>_tables.h5`` in the reference's layout (one FP64
+        dataset per table, ``/<rxn>`` and ``/<action>_<table>``;
+        dengo/chemical_network.py:310-318) when libhdf5 is available; returns
+        the path, or None without HDF5."""
+        from .tables import hdf5_available, write_tables_h5
+        if not hdf5_available():
+            return None
+        path = os.path.join(output_dir, "%s_tables.h5" % solver_name)
+        write_tables_h5(path, self.table_names(), self.table_block())
+        return path
+
     def write_solver(self, solver_name,
                      solver_template="rates_and_rate_tables_hip",
                      ode_solver_source="BE_chem_solve_hip.hip",
                      output_dir=".", init_values=None,
-                     main_name="main", input_is_number=False):
+                     main_name="main", input_is_number=False, write_h5=True):
         """Render the solver template set and write the tables.
 
-        Same signature as dengo/chemical_network.py:258-263.  Only this
-        backend's template set is available; the reference's CPU templates
-        (HDF5 + Cython) are not shipped."""
+        Same signature as dengo/chemical_network.py:258-263, plus write_h5:
+        also write the reference's HDF5 files (<name>_tables.h5 and, with
+        init_values, <name>_initial_conditions.h5) when libhdf5 is available;
+        the binary forms are always written.  Only this backend's template set
+        is available; the reference's CPU templates (HDF5 + Cython) are not
+        shipped."""
         from .codegen import render_solver
         self.input_is_number = input_is_number
         if not os.path.isdir(output_dir):
@@ -274,12 +289,20 @@ class ChemicalNetwork(object):
         written = render_solver(self, solver_name, solver_template,
                                 ode_solver_source, output_dir, main_name)
         written.append(self.write_tables(solver_name, output_dir))
+        if write_h5:
+            h5 = self.write_tables_h5(solver_name, output_dir)
+            if h5:
+                written.append(h5)
         if init_values is not None:
-            names = sorted(init_values)
-            arr = np.stack([np.asarray(init_values[n], dtype=np.float64) for n in names])
+            # dengo/chemical_network.py:320-328: one dataset per entry
+            from .tables import hdf5_available, write_fields, write_h5 as h5_write
+            ics = {k: np.asarray(init_values[k], dtype=np.float64).ravel()
+                   for k in sorted(init_values)}
             path = os.path.join(output_dir, "%s_initial_conditions.bin" % solver_name)
-            arr.astype("<f8").tofile(path)
-            with open(path[:-4] + ".json", "w") as f:
-                json.dump({"fields": names, "n": int(arr.shape[1])}, f)
+            write_fields(path, ics)
             written.append(path)
+            if write_h5 and hdf5_available():
+                path = os.path.join(output_dir, "%s_initial_conditions.h5" % solver_name)
+                h5_write(path, ics)
+                written.append(path)
         return written

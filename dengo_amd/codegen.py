@@ -49,6 +49,11 @@ def render_data_header(net, solver_name):
     w("#define %s_NSPECIES %d" % (solver_name.upper(), len(net.required_species)))
     w("#define %s_NTABLES %d" % (solver_name.upper(), len(net.table_names())))
     w("#define %s_NT %d" % (solver_name.upper(), nT))
+    w("/* Temperature grid of the tables and <name>_main's dtf")
+    w("   (rates_and_rate_tables.C.template:25-26, 110). */")
+    w("#define %s_T_LO %r" % (solver_name.upper(), float(net.T_bounds[0])))
+    w("#define %s_T_HI %r" % (solver_name.upper(), float(net.T_bounds[1])))
+    w("#define %s_STOP_TIME %r" % (solver_name.upper(), float(net.stop_time)))
     w("")
     w("typedef struct %s_data_ {" % solver_name)
     w("    double dbin;")
@@ -89,6 +94,8 @@ def render_data_header(net, solver_name):
     for i, fld in enumerate(fields):
         w("    X(%s)%s" % (fld, " \\" if i + 1 < len(fields) else ""))
     w("")
+    w("#include <stddef.h>")
+    w("")
     w('#include "dengo_types.h"')
     w("")
     w("#ifdef __cplusplus")
@@ -100,6 +107,10 @@ def render_data_header(net, solver_name):
     w("void %s_read_rate_tables(%s_data *data);" % (solver_name, solver_name))
     w("void %s_read_cooling_tables(%s_data *data);" % (solver_name, solver_name))
     w("void %s_free_data(%s_data *data);" % (solver_name, solver_name))
+    w("/* rates_and_rate_tables.C.template:56-147: reads %s_initial_conditions.h5," % solver_name)
+    w("   integrates to %s_STOP_TIME, writes %s_solution.h5 (the .bin forms of" % (solver_name.upper(), solver_name))
+    w("   dengo_amd.tables without HDF5); 0 on success, 1 on failure. */")
+    w("int %s_main(int argc, char **argv);" % solver_name)
     w("double dengo_evolve_%s(double dtf, double *dt, double z, double *input," % solver_name)
     w("        double *rtol, double *atol, long long dims, %s_data *data);" % solver_name)
     w("int calculate_rhs_%s(double *input, double *rhs, int nstrip, int nchem, void *sdata);" % solver_name)
@@ -115,6 +126,8 @@ def render_data_header(net, solver_name):
     w("int %s_set_tables(%s_data *data, const double *tables, int ntables, int nT," % (n, n))
     w("        double T0, double T1);")
     w("%s_data *%s_setup_data_from(const char *path, int *NumberOfFields, char ***FieldNames);" % (n, n))
+    w("/* sizeof(%s_data), for bindings that hold the struct as bytes. */" % n)
+    w("size_t %s_data_bytes(void);" % n)
     w("/* Integrate dims cells held in device memory (AoS [dims][NSPECIES], updated")
     w("   in place), enqueued on stream (a hipStream_t, NULL = default).  dt0 < 0")
     w("   selects dtf/1e5.  With ttot_min non-NULL the call synchronises and returns")
@@ -204,6 +217,9 @@ def render_instantiation(net, solver_name, kind=None):
           "    static constexpr int NSPEC = %s_NSPECIES;" % N,
           "    static constexpr int NTABLES = %s_NTABLES;" % N,
           "    static constexpr int NT = %s_NT;" % N,
+          "    static constexpr double T_LO = %s_T_LO;" % N,
+          "    static constexpr double T_HI = %s_T_HI;" % N,
+          "    static constexpr double STOP_TIME = %s_STOP_TIME;" % N,
           "    static const size_t *table_offsets()",
           "    {",
           "        static const size_t o[] = {",
@@ -212,6 +228,15 @@ def render_instantiation(net, solver_name, kind=None):
           "#undef X",
           "        };",
           "        return o;",
+          "    }",
+          "    static const char *const *table_names()",
+          "    {",
+          "        static const char *const t[] = {",
+          "#define X(f) #f,",
+          "            %s_TABLE_FIELDS(X)" % N,
+          "#undef X",
+          "        };",
+          "        return t;",
           "    }",
           "    static const char *const *field_names()",
           "    {",

@@ -12,6 +12,9 @@ struct Traits {
     static constexpr int NSPEC = OXYGEN_NSPECIES;
     static constexpr int NTABLES = OXYGEN_NTABLES;
     static constexpr int NT = OXYGEN_NT;
+    static constexpr double T_LO = OXYGEN_T_LO;
+    static constexpr double T_HI = OXYGEN_T_HI;
+    static constexpr double STOP_TIME = OXYGEN_STOP_TIME;
     static const size_t *table_offsets()
     {
         static const size_t o[] = {
@@ -20,6 +23,15 @@ struct Traits {
 #undef X
         };
         return o;
+    }
+    static const char *const *table_names()
+    {
+        static const char *const t[] = {
+#define X(f) #f,
+            OXYGEN_TABLE_FIELDS(X)
+#undef X
+        };
+        return t;
     }
     static const char *const *field_names()
     {

@@ -11,6 +11,9 @@ struct Traits {
     static constexpr int NSPEC = PRIMORDIAL_NSPECIES;
     static constexpr int NTABLES = PRIMORDIAL_NTABLES;
     static constexpr int NT = PRIMORDIAL_NT;
+    static constexpr double T_LO = PRIMORDIAL_T_LO;
+    static constexpr double T_HI = PRIMORDIAL_T_HI;
+    static constexpr double STOP_TIME = PRIMORDIAL_STOP_TIME;
     static const size_t *table_offsets()
     {
         static const size_t o[] = {
@@ -19,6 +22,15 @@ struct Traits {
 #undef X
         };
         return o;
+    }
+    static const char *const *table_names()
+    {
+        static const char *const t[] = {
+#define X(f) #f,
+            PRIMORDIAL_TABLE_FIELDS(X)
+#undef X
+        };
+        return t;
     }
     static const char *const *field_names()
     {

@@ -20,12 +20,14 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <string>
 #include <type_traits>
 #include <vector>
 
 #include "be_driver.h"
 #include "be_fast.h"
 #include "ma_net.h"
+#include "dengo_h5.h"
 #include "dengo_types.h"
 
 #ifndef MAX_NCELLS
@@ -287,8 +289,9 @@ inline int check_options(const dengo_options *opt, dengo_options *o)
     return 0;
 }
 
-// Traits (per network): Data (struct), NSPEC, NTABLES, NT, table_offsets(),
-// field_names(), blob_name().
+// Traits (per network): Data (struct), NSPEC, NTABLES, NT, T_LO/T_HI (the
+// temperature grid), STOP_TIME (<n>_main's dtf), table_offsets(),
+// table_names() (struct field names), field_names().
 template <class Net, class Tr>
 struct Host {
     using Data = typename Tr::Data;
@@ -423,6 +426,221 @@ struct Host {
         return 0;
     }
 
+    // The reference's <n>_tables.h5 (dengo/chemical_network.py:310-318; read
+    // by rates_and_rate_tables.C.template:278-304): one FP64 dataset of NT
+    // values per table, named after the struct field without its r_/c_
+    // prefix; the temperature grid is the network's (:25-29).
+    static int read_h5(const char *path, Data *d)
+    {
+        const dengo_h5_file f = dengo_h5_open(path, 0);
+        if (f < 0) return -1;
+        for (int t = 0; t < NTAB; ++t) {
+            char name[128];
+            snprintf(name, sizeof(name), "/%s", Tr::table_names()[t] + 2);
+            if (dengo_h5_read(f, name, (double *)table(d, t), Tr::NT) != 0) {
+                char why[256];
+                snprintf(why, sizeof(why), "%s", last_error_text());
+                dengo_h5_close(f);
+                return set_error("%s: %s", path, why);
+            }
+        }
+        dengo_h5_close(f);
+        set_grid(d, Tr::T_LO, Tr::T_HI);
+        state(d)->stale = true;
+        return 0;
+    }
+
+    // A table file of either kind, told apart by its first bytes.
+    static int read_tables(const char *path, Data *d)
+    {
+        return dengo_h5_is_hdf5(path) ? read_h5(path, d) : read_blob(path, d);
+    }
+
+    // <n>_<suffix>.h5 when it is an HDF5 file in the working directory (the
+    // reference's file) and libhdf5 can be loaded, else <n>_<suffix>.bin.
+    static const char *default_file(const char *n, const char *suffix, char *buf, size_t len)
+    {
+        snprintf(buf, len, "%s_%s.h5", n, suffix);
+        if (dengo_h5_is_hdf5(buf) && dengo_hdf5_available()) return buf;
+        snprintf(buf, len, "%s_%s.bin", n, suffix);
+        return buf;
+    }
+
+    // Field files of this backend without HDF5 ("DENGOIC1"): a 40-byte header
+    // (magic, int32 nfields, int32 0, int64 ncells, FP64 time, FP64 timestep),
+    // nfields names of 32 bytes, then FP64 [nfields][ncells].
+    static int read_fields_bin(const char *path, std::vector<std::string> &names,
+                               std::vector<double> &vals, long long &ncells, double *hdr2)
+    {
+        FILE *f = fopen(path, "rb");
+        if (!f) return set_error("cannot open %s", path);
+        char magic[8];
+        int32_t nf = 0, pad = 0;
+        long long nc = 0;
+        double tt[2];
+        const bool ok = fread(magic, 1, 8, f) == 8 && fread(&nf, 4, 1, f) == 1 &&
+                        fread(&pad, 4, 1, f) == 1 && fread(&nc, 8, 1, f) == 1 &&
+                        fread(tt, 8, 2, f) == 2 && memcmp(magic, "DENGOIC1", 8) == 0 &&
+                        nf > 0 && nf < 4096 && nc >= 0;
+        if (!ok) {
+            fclose(f);
+            return set_error("%s is not a DENGOIC1 field file", path);
+        }
+        names.assign(nf, std::string());
+        for (int i = 0; i < nf; ++i) {
+            char nm[33] = {0};
+            if (fread(nm, 1, 32, f) != 32) {
+                fclose(f);
+                return set_error("%s: short read", path);
+            }
+            names[i] = nm;
+        }
+        vals.resize((size_t)nf * nc);
+        if (nc > 0 && fread(vals.data(), 8, vals.size(), f) != vals.size()) {
+            fclose(f);
+            return set_error("%s: short read", path);
+        }
+        fclose(f);
+        ncells = nc;
+        if (hdr2) {
+            hdr2[0] = tt[0];
+            hdr2[1] = tt[1];
+        }
+        return 0;
+    }
+
+    static int write_fields_bin(const char *path, const std::vector<std::string> &names,
+                                const std::vector<const double *> &cols, long long ncells,
+                                double time, double timestep)
+    {
+        FILE *f = fopen(path, "wb");
+        if (!f) return set_error("cannot create %s", path);
+        const int32_t nf = (int32_t)names.size(), pad = 0;
+        const double tt[2] = {time, timestep};
+        bool ok = fwrite("DENGOIC1", 1, 8, f) == 8 && fwrite(&nf, 4, 1, f) == 1 &&
+                  fwrite(&pad, 4, 1, f) == 1 && fwrite(&ncells, 8, 1, f) == 1 &&
+                  fwrite(tt, 8, 2, f) == 2;
+        for (int i = 0; ok && i < nf; ++i) {
+            char nm[32] = {0};
+            strncpy(nm, names[i].c_str(), 31);
+            ok = fwrite(nm, 1, 32, f) == 32;
+        }
+        for (int i = 0; ok && i < nf; ++i)
+            ok = fwrite(cols[i], 8, (size_t)ncells, f) == (size_t)ncells;
+        fclose(f);
+        return ok ? 0 : set_error("%s: short write", path);
+    }
+
+    // Initial conditions AoS [dims][N] from <n>_initial_conditions.h5 (one
+    // dataset per species; the cell count from the energy field, as
+    // rates_and_rate_tables.C.template:63-107) or the DENGOIC1 file.
+    static int read_ics(const char *n, std::vector<double> &ics, long long &dims)
+    {
+        char buf[512];
+        const char *path = default_file(n, "initial_conditions", buf, sizeof(buf));
+        const char *const *fields = Tr::field_names();
+        if (dengo_h5_is_hdf5(path)) {
+            const dengo_h5_file f = dengo_h5_open(path, 0);
+            if (f < 0) return -1;
+            char name[128];
+            snprintf(name, sizeof(name), "/%s", fields[Net::GE]);
+            dims = dengo_h5_length(f, name);
+            if (dims < 0) {
+                dengo_h5_close(f);
+                return set_error("%s has no energy field %s", path, name);
+            }
+            ics.assign((size_t)dims * N, 0.0);
+            std::vector<double> col((size_t)dims);
+            for (int i = 0; i < N; ++i) {
+                snprintf(name, sizeof(name), "/%s", fields[i]);
+                if (dengo_h5_read(f, name, col.data(), dims) != 0) {
+                    char why[256];
+                    snprintf(why, sizeof(why), "%s", last_error_text());
+                    dengo_h5_close(f);
+                    return set_error("%s: %s", path, why);
+                }
+                for (long long j = 0; j < dims; ++j) ics[(size_t)j * N + i] = col[j];
+            }
+            dengo_h5_close(f);
+            return 0;
+        }
+        std::vector<std::string> names;
+        std::vector<double> vals;
+        if (read_fields_bin(path, names, vals, dims, nullptr) != 0) return -1;
+        ics.assign((size_t)dims * N, 0.0);
+        for (int i = 0; i < N; ++i) {
+            int k = -1;
+            for (size_t q = 0; q < names.size(); ++q)
+                if (names[q] == fields[i]) k = (int)q;
+            if (k < 0) return set_error("%s has no field %s", path, fields[i]);
+            for (long long j = 0; j < dims; ++j) ics[(size_t)j * N + i] = vals[(size_t)k * dims + j];
+        }
+        return 0;
+    }
+
+    // <n>_main (rates_and_rate_tables.C.template:56-147): tables and initial
+    // conditions from the working directory, atol = 1e-9 x IC, rtol = 1e-9,
+    // dtf = the network's stop time, dt = z = -1, dengo_evolve, then
+    // <n>_solution.h5 (.bin without HDF5): one dataset per species, /T, and
+    // the root attributes time (the returned ttot) and timestep (the final dt).
+    static int main_driver(const char *n)
+    {
+        char buf[512];
+        Data *d = setup_from(default_file(n, "tables", buf, sizeof(buf)), NULL, NULL);
+        if (!d) return 1;
+        std::vector<double> input;
+        long long dims = 0;
+        if (read_ics(n, input, dims) != 0) {
+            fprintf(stderr, "%s_main: %s\n", n, last_error_text());
+            free_data(d);
+            return 1;
+        }
+        std::vector<double> rtol(input.size(), 1e-9), atol(input.size()), T((size_t)dims);
+        for (size_t i = 0; i < input.size(); ++i) atol[i] = input[i] * 1e-9;
+        dengo_options o;
+        dengo_default_options(&o);
+        double dt = -1.0;
+        const double ttot = evolve_host(d, &o, Tr::STOP_TIME, &dt, -1.0, input.data(), rtol.data(),
+                                        atol.data(), dims, NULL, NULL, NULL, NULL, NULL, T.data());
+        free_data(d);
+        if (ttot < 0) {
+            fprintf(stderr, "%s_main: %s\n", n, last_error_text());
+            return 1;
+        }
+        std::vector<std::vector<double>> cols(N, std::vector<double>((size_t)dims));
+        for (int i = 0; i < N; ++i)
+            for (long long j = 0; j < dims; ++j) cols[i][j] = input[(size_t)j * N + i];
+        const char *const *fields = Tr::field_names();
+        int rc = 0;
+        if (dengo_hdf5_available()) {
+            snprintf(buf, sizeof(buf), "%s_solution.h5", n);
+            const dengo_h5_file f = dengo_h5_open(buf, 1);
+            if (f < 0) rc = -1;
+            char name[128];
+            for (int i = 0; rc == 0 && i < N; ++i) {
+                snprintf(name, sizeof(name), "/%s", fields[i]);
+                rc = dengo_h5_write(f, name, cols[i].data(), dims);
+            }
+            if (rc == 0) rc = dengo_h5_write(f, "/T", T.data(), dims);
+            if (rc == 0) rc = dengo_h5_set_attribute(f, "time", ttot);
+            if (rc == 0) rc = dengo_h5_set_attribute(f, "timestep", dt);
+            if (f >= 0 && dengo_h5_close(f) != 0) rc = -1;
+        } else {
+            snprintf(buf, sizeof(buf), "%s_solution.bin", n);
+            std::vector<std::string> names(fields, fields + N);
+            names.push_back("T");
+            std::vector<const double *> p;
+            for (int i = 0; i < N; ++i) p.push_back(cols[i].data());
+            p.push_back(T.data());
+            rc = write_fields_bin(buf, names, p, dims, ttot, dt);
+        }
+        if (rc != 0) {
+            fprintf(stderr, "%s_main: %s\n", n, last_error_text());
+            return 1;
+        }
+        return 0;
+    }
+
     static void field_names(int *NumberOfFields, char ***FieldNames)
     {
         if (FieldNames != NULL && NumberOfFields != NULL) {
@@ -451,7 +669,7 @@ struct Host {
     {
         Data *d = alloc();
         if (!d) return NULL;
-        if (read_blob(path, d) != 0) {
+        if (read_tables(path, d) != 0) {
             fprintf(stderr, "%s\n", last_error_text());
             free(d);
             return NULL;
@@ -880,15 +1098,30 @@ struct Host {
     {                                                                                          \
         return H::setup_from(path, nf, fn);                                                    \
     }                                                                                          \
-    n##_data *n##_setup_data(int *nf, char ***fn) { return H::setup_from(#n "_tables.bin", nf, fn); } \
+    n##_data *n##_setup_data(int *nf, char ***fn)                                              \
+    {                                                                                          \
+        char b_[512];                                                                          \
+        return H::setup_from(H::default_file(#n, "tables", b_, sizeof(b_)), nf, fn);           \
+    }                                                                                          \
     void n##_read_rate_tables(n##_data *d)                                                     \
     {                                                                                          \
-        if (H::read_blob(#n "_tables.bin", d) != 0) fprintf(stderr, "%s\n", dengo_last_error()); \
+        char b_[512];                                                                          \
+        if (H::read_tables(H::default_file(#n, "tables", b_, sizeof(b_)), d) != 0)             \
+            fprintf(stderr, "%s\n", dengo_last_error());                                       \
     }                                                                                          \
     void n##_read_cooling_tables(n##_data *d)                                                  \
     {                                                                                          \
-        if (H::read_blob(#n "_tables.bin", d) != 0) fprintf(stderr, "%s\n", dengo_last_error()); \
+        char b_[512];                                                                          \
+        if (H::read_tables(H::default_file(#n, "tables", b_, sizeof(b_)), d) != 0)             \
+            fprintf(stderr, "%s\n", dengo_last_error());                                       \
     }                                                                                          \
+    int n##_main(int argc, char **argv)                                                        \
+    {                                                                                          \
+        (void)argc;                                                                            \
+        (void)argv;                                                                            \
+        return H::main_driver(#n);                                                             \
+    }                                                                                          \
+    size_t n##_data_bytes(void) { return sizeof(n##_data); }                                   \
     void n##_free_data(n##_data *d) { H::free_data(d); }                                       \
     int dengo_##n##_be_chem_solve(n##_data *d, double *u, double dt, double *rt, double *at,   \
                                   int nstrip, int nchem, double *scaling)                      \

@@ -104,6 +104,8 @@ def _network_signatures(n):
         "ensure_electron_consistency_%s" % n: (None, [_dp, ctypes.c_int, ctypes.c_int]),
         "dengo_%s_set_current_z" % n: (None, [ctypes.c_void_p, ctypes.c_double]),
         "dengo_%s_strip_temperatures" % n: (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int]),
+        "%s_main" % n: (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p]),
+        "%s_data_bytes" % n: (ctypes.c_size_t, []),
     }
 
 
@@ -111,6 +113,17 @@ COMMON_SIGNATURES = {
     "dengo_default_options": (None, [ctypes.POINTER(Options)]),
     "dengo_last_error": (ctypes.c_char_p, []),
     "dengo_last_kernel_ms": (ctypes.c_double, []),
+    "dengo_hdf5_available": (ctypes.c_int, []),
+    "dengo_h5_open": (ctypes.c_longlong, [ctypes.c_char_p, ctypes.c_int]),
+    "dengo_h5_close": (ctypes.c_int, [ctypes.c_longlong]),
+    "dengo_h5_length": (ctypes.c_longlong, [ctypes.c_longlong, ctypes.c_char_p]),
+    "dengo_h5_read": (ctypes.c_int, [ctypes.c_longlong, ctypes.c_char_p, _dp, ctypes.c_longlong]),
+    "dengo_h5_write": (ctypes.c_int, [ctypes.c_longlong, ctypes.c_char_p, _dp,
+                                      ctypes.c_longlong]),
+    "dengo_h5_set_attribute": (ctypes.c_int, [ctypes.c_longlong, ctypes.c_char_p,
+                                              ctypes.c_double]),
+    "dengo_h5_get_attribute": (ctypes.c_int, [ctypes.c_longlong, ctypes.c_char_p, _dp]),
+    "dengo_h5_is_hdf5": (ctypes.c_int, [ctypes.c_char_p]),
 }
 
 

@@ -10,6 +10,11 @@
 #define OXYGEN_NSPECIES 11
 #define OXYGEN_NTABLES 25
 #define OXYGEN_NT 1024
+/* Temperature grid of the tables and <name>_main's dtf
+   (rates_and_rate_tables.C.template:25-26, 110). */
+#define OXYGEN_T_LO 10000.0
+#define OXYGEN_T_HI 1000000000.0
+#define OXYGEN_STOP_TIME 1000000000000.0
 
 typedef struct oxygen_data_ {
     double dbin;
@@ -140,6 +145,8 @@ typedef struct oxygen_data_ {
     X(c_O_8_c_O_8_c) \
     X(c_O_9_c_O_9_c)
 
+#include <stddef.h>
+
 #include "dengo_types.h"
 
 #ifdef __cplusplus
@@ -151,6 +158,10 @@ oxygen_data *oxygen_setup_data(int *NumberOfFields, char ***FieldNames);
 void oxygen_read_rate_tables(oxygen_data *data);
 void oxygen_read_cooling_tables(oxygen_data *data);
 void oxygen_free_data(oxygen_data *data);
+/* rates_and_rate_tables.C.template:56-147: reads oxygen_initial_conditions.h5,
+   integrates to OXYGEN_STOP_TIME, writes oxygen_solution.h5 (the .bin forms of
+   dengo_amd.tables without HDF5); 0 on success, 1 on failure. */
+int oxygen_main(int argc, char **argv);
 double dengo_evolve_oxygen(double dtf, double *dt, double z, double *input,
         double *rtol, double *atol, long long dims, oxygen_data *data);
 int calculate_rhs_oxygen(double *input, double *rhs, int nstrip, int nchem, void *sdata);
@@ -165,6 +176,8 @@ oxygen_data *oxygen_alloc_data(void);
 int oxygen_set_tables(oxygen_data *data, const double *tables, int ntables, int nT,
         double T0, double T1);
 oxygen_data *oxygen_setup_data_from(const char *path, int *NumberOfFields, char ***FieldNames);
+/* sizeof(oxygen_data), for bindings that hold the struct as bytes. */
+size_t oxygen_data_bytes(void);
 /* Integrate dims cells held in device memory (AoS [dims][NSPECIES], updated
    in place), enqueued on stream (a hipStream_t, NULL = default).  dt0 < 0
    selects dtf/1e5.  With ttot_min non-NULL the call synchronises and returns

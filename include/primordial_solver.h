@@ -10,6 +10,11 @@
 #define PRIMORDIAL_NSPECIES 10
 #define PRIMORDIAL_NTABLES 45
 #define PRIMORDIAL_NT 1024
+/* Temperature grid of the tables and <name>_main's dtf
+   (rates_and_rate_tables.C.template:25-26, 110). */
+#define PRIMORDIAL_T_LO 1.0
+#define PRIMORDIAL_T_HI 1000000000.0
+#define PRIMORDIAL_STOP_TIME 1000000000000.0
 
 typedef struct primordial_data_ {
     double dbin;
@@ -220,6 +225,8 @@ typedef struct primordial_data_ {
     X(c_reHeIII_reHeIII) \
     X(c_reHII_reHII)
 
+#include <stddef.h>
+
 #include "dengo_types.h"
 
 #ifdef __cplusplus
@@ -231,6 +238,10 @@ primordial_data *primordial_setup_data(int *NumberOfFields, char ***FieldNames);
 void primordial_read_rate_tables(primordial_data *data);
 void primordial_read_cooling_tables(primordial_data *data);
 void primordial_free_data(primordial_data *data);
+/* rates_and_rate_tables.C.template:56-147: reads primordial_initial_conditions.h5,
+   integrates to PRIMORDIAL_STOP_TIME, writes primordial_solution.h5 (the .bin forms of
+   dengo_amd.tables without HDF5); 0 on success, 1 on failure. */
+int primordial_main(int argc, char **argv);
 double dengo_evolve_primordial(double dtf, double *dt, double z, double *input,
         double *rtol, double *atol, long long dims, primordial_data *data);
 int calculate_rhs_primordial(double *input, double *rhs, int nstrip, int nchem, void *sdata);
@@ -245,6 +256,8 @@ primordial_data *primordial_alloc_data(void);
 int primordial_set_tables(primordial_data *data, const double *tables, int ntables, int nT,
         double T0, double T1);
 primordial_data *primordial_setup_data_from(const char *path, int *NumberOfFields, char ***FieldNames);
+/* sizeof(primordial_data), for bindings that hold the struct as bytes. */
+size_t primordial_data_bytes(void);
 /* Integrate dims cells held in device memory (AoS [dims][NSPECIES], updated
    in place), enqueued on stream (a hipStream_t, NULL = default).  dt0 < 0
    selects dtf/1e5.  With ttot_min non-NULL the call synchronises and returns

@@ -10,6 +10,11 @@
 #define CARBON_NSPECIES 9
 #define CARBON_NTABLES 19
 #define CARBON_NT 1024
+/* Temperature grid of the tables and <name>_main's dtf
+   (rates_and_rate_tables.C.template:25-26, 110). */
+#define CARBON_T_LO 10000.0
+#define CARBON_T_HI 1000000000.0
+#define CARBON_STOP_TIME 1000000000000.0
 
 typedef struct carbon_data_ {
     double dbin;
@@ -116,6 +121,8 @@ typedef struct carbon_data_ {
     X(c_C_6_c_C_6_c) \
     X(c_C_7_c_C_7_c)
 
+#include <stddef.h>
+
 #include "dengo_types.h"
 
 #ifdef __cplusplus
@@ -127,6 +134,10 @@ carbon_data *carbon_setup_data(int *NumberOfFields, char ***FieldNames);
 void carbon_read_rate_tables(carbon_data *data);
 void carbon_read_cooling_tables(carbon_data *data);
 void carbon_free_data(carbon_data *data);
+/* rates_and_rate_tables.C.template:56-147: reads carbon_initial_conditions.h5,
+   integrates to CARBON_STOP_TIME, writes carbon_solution.h5 (the .bin forms of
+   dengo_amd.tables without HDF5); 0 on success, 1 on failure. */
+int carbon_main(int argc, char **argv);
 double dengo_evolve_carbon(double dtf, double *dt, double z, double *input,
         double *rtol, double *atol, long long dims, carbon_data *data);
 int calculate_rhs_carbon(double *input, double *rhs, int nstrip, int nchem, void *sdata);
@@ -141,6 +152,8 @@ carbon_data *carbon_alloc_data(void);
 int carbon_set_tables(carbon_data *data, const double *tables, int ntables, int nT,
         double T0, double T1);
 carbon_data *carbon_setup_data_from(const char *path, int *NumberOfFields, char ***FieldNames);
+/* sizeof(carbon_data), for bindings that hold the struct as bytes. */
+size_t carbon_data_bytes(void);
 /* Integrate dims cells held in device memory (AoS [dims][NSPECIES], updated
    in place), enqueued on stream (a hipStream_t, NULL = default).  dt0 < 0
    selects dtf/1e5.  With ttot_min non-NULL the call synchronises and returns

@@ -12,6 +12,9 @@ struct Traits {
     static constexpr int NSPEC = CARBON_NSPECIES;
     static constexpr int NTABLES = CARBON_NTABLES;
     static constexpr int NT = CARBON_NT;
+    static constexpr double T_LO = CARBON_T_LO;
+    static constexpr double T_HI = CARBON_T_HI;
+    static constexpr double STOP_TIME = CARBON_STOP_TIME;
     static const size_t *table_offsets()
     {
         static const size_t o[] = {
@@ -20,6 +23,15 @@ struct Traits {
 #undef X
         };
         return o;
+    }
+    static const char *const *table_names()
+    {
+        static const char *const t[] = {
+#define X(f) #f,
+            CARBON_TABLE_FIELDS(X)
+#undef X
+        };
+        return t;
     }
     static const char *const *field_names()
     {

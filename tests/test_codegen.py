@@ -8,7 +8,7 @@ import numpy as np
 
 from dengo_amd import codegen, runtime
 from dengo_amd.networks import ion_network, oxygen_network, primordial_network
-from dengo_amd.tables import read_blob
+from dengo_amd.tables import hdf5_available, read_blob
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "dengo_amd", "csrc")
@@ -34,8 +34,11 @@ def test_write_solver_emits_template_set(tmp_path):
     net = ion_network("C", seed=9)
     written = net.write_solver("carbon", output_dir=str(tmp_path))
     names = sorted(os.path.basename(p) for p in written)
-    assert names == sorted(["carbon_solver.h", "net_carbon.h", "carbon_solver.hip",
-                            "carbon_run.py", "carbon_tables.bin"])
+    expect = ["carbon_solver.h", "net_carbon.h", "carbon_solver.hip", "carbon_run.py",
+              "carbon_tables.bin"]
+    if hdf5_available():
+        expect.append("carbon_tables.h5")
+    assert names == sorted(expect)
     for f in ("carbon_solver.h", "net_carbon.h", "carbon_solver.hip", "carbon_run.py",
               "carbon_tables.json"):
         assert _read(str(tmp_path), f) == _read(CARBON, f), f
