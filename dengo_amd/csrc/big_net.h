@@ -79,6 +79,22 @@ DENGO_HD unsigned jac_op(int row, int lc, int nu, int partner, int r)
            ((unsigned)partner << 14) | ((unsigned)r << 21);
 }
 
+// Diagonal-entry operation: bits 0-7 row (N: idle thread), 8-10 change + 4,
+// 11-17 partner species, 18-28 reaction.
+DENGO_HD unsigned diag_op(int row, int nu, int partner, int r)
+{
+    return (unsigned)row | ((unsigned)(nu + 4) << 8) | ((unsigned)partner << 11) |
+           ((unsigned)r << 18);
+}
+// Owner operation of the register-resident Jacobian (one lane's entry of its
+// wave's tile column): bits 5-7 change + 4, 8-14 partner species, 15-25
+// reaction.
+DENGO_HD unsigned own_op(int nu, int partner, int r)
+{
+    return ((unsigned)(nu + 4) << 5) | ((unsigned)partner << 8) | ((unsigned)r << 15);
+}
+constexpr int OWN_PREFETCH = 8;        // operations in flight per lane
+
 struct NetDev {
     int nrxn;
     const int4 *rxn;                   // (a, b, c, d)
@@ -88,12 +104,21 @@ struct NetDev {
     const unsigned *jac_ops;           // chunk J: [jac_len[J]][NTHR] at jac_off[J] * NTHR
     int jac_len[NT];
     int jac_off[NT];
+    // the Newton matrix's off-diagonal entries, summed by their owner lanes:
+    // wave w's program is [rows][64] at jac_ops + own_off[w], slot s's block
+    // (rows own_cnt[32 w + s], read at jac_ops + own_cnt_off) after the
+    // blocks of slots < s, then OWN_PREFETCH rows of padding; the diagonal
+    // entries are gathered into LDS by [diag_len][NTHR] at jac_ops +
+    // diag_off * NTHR
+    long long own_off[NWAVE];
+    long long own_cnt_off;
+    int diag_len, diag_off;
 };
 
 struct Shared {
     double k[MAX_RXN];
-    double buf[BUF_SZ];                // fluxes; a Jacobian chunk; a panel's multipliers
-    double buf2[CHUNK_SZ];             // the other Jacobian chunk buffer
+    double buf[BUF_SZ];                // fluxes; a panel's multipliers
+    double jd[N];                      // the Jacobian's diagonal entries
     double u[N], u0[N], sc[N], inv[N], prev[N], at[N], rt[N], x[N], s[N], pv[N];
     double f[N + 1];
     unsigned short ab[MAX_RXN];        // a | b << 8
@@ -256,20 +281,19 @@ __device__ __forceinline__ void run_jac_chunk(const NetDev &net, Shared &S, cons
     if (e_cur >= 0) S.buf[e_cur] = acc;
 }
 
-// The BE matrix M = I - dt diag(inv) Jf diag(sc) (BE_Resid_Jac, :280-321)
-// from the 8 Jacobian column chunks.  Chunk J is gathered into one of two
-// LDS buffers while the owner wave of chunk J-1 scales the other one into
-// its registers: one barrier per chunk.  Per entry the operations are those
-// of run_jac_chunk, in the same order.
+// The BE matrix M = I - dt diag(inv) Jf diag(sc) (BE_Resid_Jac, :280-321).
+// The diagonal entries (the longest sums) are gathered into LDS by all
+// threads; every other entry is summed by the lane that holds it, straight
+// into its register, from a per-wave program whose entry boundaries fall on
+// the same operation in every lane.  Per entry the operations, their order
+// and the scaling are those of run_jac_chunk and the reference.
 __device__ __forceinline__ void assemble_jacobian(const NetDev &net, Shared &S, const Geo &G,
                                                   double (&M)[TCW][RPL][TILE], double mdt)
 {
     const unsigned t = G.t;
-    static_for<0, NT>([&](auto JC) {
-        constexpr int J = CV(JC);
-        double *cb = (J & 1) ? S.buf2 : S.buf;
-        const unsigned *ops = net.jac_ops + (size_t)net.jac_off[J] * NTHR;
-        const int L = net.jac_len[J];
+    {
+        const unsigned *ops = net.jac_ops + (size_t)net.diag_off * NTHR;
+        const int L = net.diag_len;
         unsigned cur[OPS_BATCH], nxt[OPS_BATCH];
         if (L > 0) load_ops(ops, t, 0, cur);
         double acc = 0.0;
@@ -279,10 +303,11 @@ __device__ __forceinline__ void assemble_jacobian(const NetDev &net, Shared &S, 
 #pragma unroll
             for (int i = 0; i < OPS_BATCH; ++i) {
                 const unsigned o = cur[i];
-                const int e = (int)((o >> 7) & 15) * CHUNK_LD + (int)(o & 127);
-                const int nu = (int)((o >> 11) & 7) - 4, partner = (o >> 14) & 127, r = o >> 21;
+                const int e = (int)(o & 0xff);
+                const int nu = (int)((o >> 8) & 7) - 4, partner = (o >> 11) & 127,
+                          r = (o >> 18) & 2047;
                 if (e != e_cur) {
-                    if (e_cur >= 0) cb[e_cur] = acc;
+                    if (e_cur >= 0 && e_cur < N) S.jd[e_cur] = acc;
                     acc = 0.0;
                     e_cur = e;
                 }
@@ -294,30 +319,49 @@ __device__ __forceinline__ void assemble_jacobian(const NetDev &net, Shared &S, 
 #pragma unroll
             for (int i = 0; i < OPS_BATCH; ++i) cur[i] = nxt[i];
         }
-        if (e_cur >= 0) cb[e_cur] = acc;
-        __syncthreads();
-        if (G.w == J % NWAVE) {
-            constexpr int TC = J / NWAVE;
-            double iv[RPL];
+        if (e_cur >= 0 && e_cur < N) S.jd[e_cur] = acc;
+    }
+    // this lane's rows and their inverse scales
+    double iv[RPL];
 #pragma unroll
-            for (int h = 0; h < RPL; ++h) iv[h] = S.inv[G.row(h)];
+    for (int h = 0; h < RPL; ++h) iv[h] = S.inv[G.row(h)];
+    __syncthreads();
+    {
+        const int w = __builtin_amdgcn_readfirstlane(G.w);
+        const unsigned *ops = net.jac_ops + net.own_off[w];
+        const unsigned *cnt = net.jac_ops + net.own_cnt_off + RPL * TILE * w;
+        const unsigned lane = (unsigned)G.lane;
+        // the operation stream, OWN_PREFETCH rows ahead in a register ring
+        unsigned ring[OWN_PREFETCH];
 #pragma unroll
-            for (int c = 0; c < TILE; ++c) {
-                const int colg = TILE * J + c;
-                const double scc = S.sc[colg];
+        for (int i = 0; i < OWN_PREFETCH; ++i) ring[i] = ops[lane + (unsigned)i * 64];
+        unsigned next = OWN_PREFETCH;
+        static_for<0, RPL * TILE>([&](auto SL) {
+            constexpr int h = CV(SL) / TILE, c = CV(SL) % TILE;
+            const int K = (int)cnt[CV(SL)];
+            double acc = 0.0;
+            for (int q = 0; q < K; ++q) {
+                const unsigned o = ring[0];
 #pragma unroll
-                for (int h = 0; h < RPL; ++h) {
-                    const int row = G.row(h);
-                    double val = cb[c * CHUNK_LD + row];
-                    val = val * iv[h];
-                    val = val * scc;
-                    val = mdt * val;
-                    if (row == colg) val = val + 1.0;
-                    M[TC][h][c] = val;
+                for (int i = 0; i + 1 < OWN_PREFETCH; ++i) ring[i] = ring[i + 1];
+                ring[OWN_PREFETCH - 1] = ops[lane + next * 64];
+                ++next;
+                const int nu = (int)((o >> 5) & 7) - 4, partner = (o >> 8) & 127,
+                          r = (o >> 15) & 2047;
+                if (nu != 0) {
+                    const double part = S.k[r] * S.u[partner];
+                    acc = acc + (double)nu * part;
                 }
             }
-        }
-    });
+            const int row = G.row(h), colg = TILE * w + c;
+            double val = row == colg ? S.jd[row] : acc;
+            val = val * iv[h];
+            val = val * S.sc[colg];
+            val = mdt * val;
+            if (row == colg) val = val + 1.0;
+            M[0][h][c] = val;
+        });
+    }
     __syncthreads();
 }
 

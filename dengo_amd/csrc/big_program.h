@@ -16,6 +16,9 @@ struct Programs {
     std::vector<unsigned> rhs_ops, jac_ops;
     int rhs_len = 0;
     int jac_len[NT] = {}, jac_off[NT] = {};
+    long long own_off[NWAVE] = {};
+    long long own_cnt_off = 0;
+    int diag_len = 0, diag_off = 0;
 };
 
 // Distributes per-entry operation lists over the threads of a cell, longest
@@ -105,6 +108,57 @@ inline Programs build_programs(const int *rxn, int nrxn)
         // every thread gets at least one of the 2048 entries
         pack(lists, 0u, jac_pad, P.jac_ops, P.jac_len[J]);
     }
+    // Diagonal entries (the longest lists: one operation per reaction the
+    // species is a reactant of), balanced over the cell's threads.
+    {
+        std::vector<std::vector<unsigned>> lists(N);
+        for (int row = 0; row < N; ++row) {
+            for (unsigned o : ent[(size_t)row * N + row])
+                lists[row].push_back(diag_op(row, (int)((o >> 11) & 7) - 4, (o >> 14) & 127,
+                                             (int)(o >> 21)));
+            if (lists[row].empty()) lists[row].push_back(diag_op(row, 0, 0, 0));
+        }
+        auto diag_pad = [](unsigned o) { return diag_op((int)(o & 0xff), 0, 0, 0); };
+        P.diag_off = (int)(P.jac_ops.size() / NTHR);
+        pack(lists, diag_op(N, 0, 0, 0), diag_pad, P.jac_ops, P.diag_len);
+    }
+    // Off-diagonal entries, owner-computed: lane l of wave w sums entry (row
+    // l + 64 h, column TILE w + c) for slot h TILE + c, slots in order; every
+    // slot's block has as many rows as the longest list among the wave's
+    // lanes (possibly none), shorter lists padded with no-ops.  Each entry
+    // keeps its list's order.
+    static_assert(TCW == 1, "one tile column per wave");
+    std::vector<unsigned> counts;
+    for (int w = 0; w < NWAVE; ++w) {
+        std::vector<unsigned> prog;
+        for (int slot = 0; slot < RPL * TILE; ++slot) {
+            const int h = slot / TILE, c = slot % TILE, col = TILE * w + c;
+            size_t K = 0;
+            for (int l = 0; l < 64; ++l) {
+                const int row = l + 64 * h;
+                if (row != col) K = std::max(K, ent[(size_t)col * N + row].size());
+            }
+            counts.push_back((unsigned)K);
+            for (size_t q = 0; q < K; ++q) {
+                for (int l = 0; l < 64; ++l) {
+                    const int row = l + 64 * h;
+                    const std::vector<unsigned> &e = ent[(size_t)col * N + row];
+                    unsigned op = own_op(0, 0, 0);
+                    if (row != col && q < e.size()) {
+                        const unsigned o = e[q];
+                        op = own_op((int)((o >> 11) & 7) - 4, (o >> 14) & 127, (int)(o >> 21));
+                    }
+                    prog.push_back(op);
+                }
+            }
+        }
+        // rows the prefetch ring reads past the last block
+        prog.insert(prog.end(), (size_t)OWN_PREFETCH * 64, own_op(0, 0, 0));
+        P.own_off[w] = (long long)P.jac_ops.size();
+        P.jac_ops.insert(P.jac_ops.end(), prog.begin(), prog.end());
+    }
+    P.own_cnt_off = (long long)P.jac_ops.size();
+    P.jac_ops.insert(P.jac_ops.end(), counts.begin(), counts.end());
     return P;
 }
 

@@ -85,6 +85,10 @@ B::NetDev net_view(const synthetic_data *d)
         v.jac_len[J] = d->prog.jac_len[J];
         v.jac_off[J] = d->prog.jac_off[J];
     }
+    for (int w = 0; w < B::NWAVE; ++w) v.own_off[w] = d->prog.own_off[w];
+    v.own_cnt_off = d->prog.own_cnt_off;
+    v.diag_len = d->prog.diag_len;
+    v.diag_off = d->prog.diag_off;
     return v;
 }
 
