@@ -482,7 +482,13 @@ __device__ __forceinline__ void factor_panel(double (&M)[TCW][RPL][TILE], Shared
         // multipliers of this wave's rows (published for the other waves) and
         // the right-hand side
         double rp = 0.0;
-        if constexpr (MODE == kGJ) rp = 1.0 / piv;
+        if constexpr (MODE == kGJ) {
+            // hardware reciprocal estimate and two Newton-Raphson steps
+            // (r <- r + r (1 - piv r)): the division is on the pivot chain
+            rp = __builtin_amdgcn_rcp(piv);
+            rp = fma(rp, fma(-piv, rp, 1.0), rp);
+            rp = fma(rp, fma(-piv, rp, 1.0), rp);
+        }
         double l[RPL];
 #pragma unroll
         for (int h = 0; h < RPL; ++h) {
@@ -507,10 +513,13 @@ __device__ __forceinline__ void factor_panel(double (&M)[TCW][RPL][TILE], Shared
         // pivot row right of the pivot column, and the panel's own update
         double u[TILE];
         const int lq = p & 63;
+        // hp is wave-uniform: one branch, then plain readlanes
+        if (hp) {
 #pragma unroll
-        for (int c = 0; c < TILE; ++c) {
-            u[c] = 0.0;
-            if (c > kk) u[c] = readlane_d(hp ? M[TC][1][c] : M[TC][0][c], lq);
+            for (int c = 0; c < TILE; ++c) u[c] = c > kk ? readlane_d(M[TC][1][c], lq) : 0.0;
+        } else {
+#pragma unroll
+            for (int c = 0; c < TILE; ++c) u[c] = c > kk ? readlane_d(M[TC][0][c], lq) : 0.0;
         }
         if constexpr (MODE == kLU) {
             if (G.lane == 0) {
