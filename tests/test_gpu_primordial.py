@@ -114,6 +114,33 @@ def test_edge_cases(solver, prim_tables):
     assert np.array_equal(r["nsub"], o["nsub"]) and np.array_equal(r["nrej"], o["nrej"])
 
 
+def test_out_of_range_and_invalid_inputs(solver, prim_tables):
+    """Temperatures beyond the table bounds (the clamp and the top bin's
+    unclamped-t1 interpolation), a NaN species and a negative species: the
+    faithful kernels reproduce the oracle bit for bit on every cell, both
+    schedules; the fast kernel gives up on the same invalid cells."""
+    _, tab = prim_tables
+    inp, _, _ = primordial_ics(48, seed=77)
+    x = inp.copy()
+    x[0:8, 9] *= 1e4        # T far above 1e9 K
+    x[8:16, 9] *= 1e-6      # T below 1 K
+    x[16:20, 2] = np.nan    # NaN HI
+    x[20:24, 3] = -x[20:24, 3]   # negative HII
+    rtol, atol = tolerances(inp)
+    o = oracle.prim_evolve(x, rtol, atol, tab, 1e10, flags=oracle.PORTABLE_LOG, max_substeps=5000)
+    print("out-of-range cells: status", np.bincount(o["status"], minlength=3).tolist(), flush=True)
+    for sched in (runtime.PER_CELL, runtime.PERSISTENT):
+        r = solver.evolve(x, rtol, atol, 1e10, schedule=sched, max_substeps=5000)
+        for k in ("status", "nsub", "nrej", "nsweep"):
+            assert np.array_equal(r[k], o[k]), k
+        # NaN states compare equal position by position
+        assert np.array_equal(r["state"], o["state"], equal_nan=True)
+        assert np.array_equal(r["atol"], o["atol"], equal_nan=True)
+        assert r["ttot"] == o["ttot"] or (np.isnan(r["ttot"]) and np.isnan(o["ttot"]))
+    f = solver.evolve(x, rtol, atol, 1e10, precision=runtime.FP64_FAST, max_substeps=5000)
+    assert np.all(f["status"][16:24] == 1) and np.all(o["status"][16:24] == 1)
+
+
 def test_drop_in_entry_point(tmp_path, prim_tables):
     """primordial_setup_data reads primordial_tables.bin from the working
     directory and dengo_evolve_primordial updates host arrays in place."""
