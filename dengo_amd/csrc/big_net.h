@@ -113,6 +113,7 @@ struct NetDev {
     long long own_off[NWAVE];
     long long own_cnt_off;
     int diag_len, diag_off;
+    const long long *order;            // nullptr, or the hand-out order of the cells
 };
 
 struct Shared {
@@ -777,6 +778,15 @@ __global__ __launch_bounds__(NTHR) void k_big(NetDev net, double dtf, double dt_
             cell = blockIdx.x;
         }
         if (cell >= ncells) break;
+        if (net.order) {
+            cell = net.order[cell];
+            if (cell < 0 || cell >= ncells) {
+                // not reached with a checked order; every thread has read
+                // S.cell before thread 0 fetches the next one
+                __syncthreads();
+                continue;
+            }
+        }
         const double T = Tc[cell];
         if (t < N) {
             const double v = X[cell * N + t];

@@ -52,6 +52,8 @@ dengo_graph *dengo_graph_create(int network, void *data, const dengo_options *op
     o.block = 0;
     int full = 0;
     if (h.prepare(data, &o, &full)) return nullptr;   // tables resident before capture
+    // a cell order is checked once, here; replays use the same array
+    if (o.order && h.prepare_order(data, o.order, dims, nullptr)) return nullptr;
     dengo_graph *g = new dengo_graph();
     hipStream_t st = nullptr;
     auto fail = [&](const char *what, hipError_t e) -> dengo_graph * {

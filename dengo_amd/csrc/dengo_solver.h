@@ -251,11 +251,142 @@ struct NetHooks {
     int nspecies;
     int isothermal;               // 1: cells carry T[n] instead of an energy unknown
     int (*prepare)(void *data, const dengo_options *opt, int *full_grid);
+    // buffers for and check of a cell order of n cells (synchronises stream)
+    int (*prepare_order)(void *data, const long long *order, long long n, void *stream);
     int (*launch)(void *data, const LaunchArgs &a);
 };
 NetHooks hooks_primordial();
 NetHooks hooks_oxygen();
 NetHooks hooks_synthetic();
+
+// ---------------------------------------------------------------------------
+// Cell order (dengo_options.order).  The lane-per-cell networks gather the
+// cells into the order's sequence before a launch and scatter them back
+// after it, so their kernels see contiguous cells and need no indirection;
+// the order must be a permutation of 0..dims-1, which order_check verifies
+// (one pass and one synchronisation) before anything is launched.
+// ---------------------------------------------------------------------------
+struct OrderBufs {
+    long long cap = 0;                 // cells
+    int n = 0;                         // unknowns per cell of the row buffers
+    double *in = nullptr, *rt = nullptr, *at = nullptr;   // [cap][n]
+    double *dt = nullptr, *ttot = nullptr, *T = nullptr;  // [cap]
+    int *status = nullptr, *nsub = nullptr, *nrej = nullptr, *nsweep = nullptr;
+    int *seen = nullptr;
+    unsigned long long *bad = nullptr;
+};
+
+template <int DUMMY>
+__global__ void k_order_check(const long long *order, long long n, int *seen,
+                              unsigned long long *bad)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long j = order[i];
+    if (j < 0 || j >= n || atomicExch(&seen[j], 1) != 0) atomicAdd(bad, 1ull);
+}
+
+// dst row i = src row order[i] (gather) or dst row order[i] = src row i
+// (scatter), rows of N values, for up to three arrays at once.
+template <int N, bool GATHER>
+__global__ void k_order_rows(const long long *order, long long n, const double *a0,
+                             const double *a1, const double *a2, double *b0, double *b1,
+                             double *b2)
+{
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * N) return;
+    const long long i = e / N, k = e - i * N;
+    const long long o = order[i] * N + k;
+    const long long from = GATHER ? o : e, to = GATHER ? e : o;
+    if (a0) b0[to] = a0[from];
+    if (a1) b1[to] = a1[from];
+    if (a2) b2[to] = a2[from];
+}
+
+// Per-cell outputs: dst[order[i]] = src[i] for each field dst requests.
+template <int DUMMY>
+__global__ void k_order_outputs(const long long *order, long long n, dengo_cell_outputs src,
+                                dengo_cell_outputs dst)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long j = order[i];
+    if (dst.status) dst.status[j] = src.status[i];
+    if (dst.nsub) dst.nsub[j] = src.nsub[i];
+    if (dst.nrej) dst.nrej[j] = src.nrej[i];
+    if (dst.nsweep) dst.nsweep[j] = src.nsweep[i];
+    if (dst.dt) dst.dt[j] = src.dt[i];
+    if (dst.ttot) dst.ttot[j] = src.ttot[i];
+    if (dst.T) dst.T[j] = src.T[i];
+}
+
+template <class T>
+inline int order_grow(T **p, long long n)
+{
+    if (*p) DENGO_HIP_OK(hipFree(*p));
+    *p = nullptr;
+    DENGO_HIP_OK(hipMalloc(p, (size_t)(n > 0 ? n : 1) * sizeof(T)));
+    return 0;
+}
+
+// Buffers for n cells of nrow unknowns (nrow 0: the permutation check only).
+inline int order_alloc(OrderBufs &b, long long n, int nrow)
+{
+    if (!b.bad) DENGO_HIP_OK(hipMalloc(&b.bad, sizeof(unsigned long long)));
+    if (n <= b.cap && nrow <= b.n) return 0;
+    const long long cap = n > b.cap ? n : b.cap;
+    const int nr = nrow > b.n ? nrow : b.n;
+    if (order_grow(&b.seen, cap)) return -1;
+    if (nr > 0 &&
+        (order_grow(&b.in, cap * nr) || order_grow(&b.rt, cap * nr) || order_grow(&b.at, cap * nr) ||
+         order_grow(&b.dt, cap) || order_grow(&b.ttot, cap) || order_grow(&b.T, cap) ||
+         order_grow(&b.status, cap) || order_grow(&b.nsub, cap) || order_grow(&b.nrej, cap) ||
+         order_grow(&b.nsweep, cap)))
+        return -1;
+    b.cap = cap;
+    b.n = nr;
+    return 0;
+}
+
+inline void order_free(OrderBufs &b)
+{
+    void *ptrs[] = {b.in, b.rt, b.at, b.dt, b.ttot, b.T, b.status, b.nsub, b.nrej, b.nsweep,
+                    b.seen, b.bad};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    b = OrderBufs();
+}
+
+// Fails unless order[0..n) is a permutation of 0..n-1 (synchronises stream).
+inline int order_check(OrderBufs &b, const long long *order, long long n, hipStream_t st)
+{
+    if (n <= 0) return 0;
+    DENGO_HIP_OK(hipMemsetAsync(b.seen, 0, (size_t)n * sizeof(int), st));
+    DENGO_HIP_OK(hipMemsetAsync(b.bad, 0, sizeof(unsigned long long), st));
+    hipLaunchKernelGGL(k_order_check<0>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, order,
+                       n, b.seen, b.bad);
+    DENGO_HIP_OK(hipGetLastError());
+    unsigned long long bad = 0;
+    DENGO_HIP_OK(hipMemcpyAsync(&bad, b.bad, sizeof(bad), hipMemcpyDeviceToHost, st));
+    DENGO_HIP_OK(hipStreamSynchronize(st));
+    if (bad) return set_error("order is not a permutation of 0..%lld (%llu entries repeat or lie "
+                              "outside)", n - 1, bad);
+    return 0;
+}
+
+// The per-cell output fields out requests, in the order buffers.
+inline dengo_cell_outputs order_outputs(const OrderBufs &b, const dengo_cell_outputs &out)
+{
+    dengo_cell_outputs p;
+    p.status = out.status ? b.status : nullptr;
+    p.nsub = out.nsub ? b.nsub : nullptr;
+    p.nrej = out.nrej ? b.nrej : nullptr;
+    p.nsweep = out.nsweep ? b.nsweep : nullptr;
+    p.dt = out.dt ? b.dt : nullptr;
+    p.ttot = out.ttot ? b.ttot : nullptr;
+    p.T = out.T ? b.T : nullptr;
+    return p;
+}
 
 struct DevState {
     int device = -1;
@@ -273,6 +404,7 @@ struct DevState {
     long long strip_cap = 0;
     double *d_u0 = nullptr, *d_sc = nullptr, *d_inv = nullptr, *d_unew = nullptr;
     int *d_code = nullptr;
+    OrderBufs ord;                     // cell order: rows and outputs in hand-out order
 };
 
 inline int check_options(const dengo_options *opt, dengo_options *o)
@@ -690,6 +822,7 @@ struct Host {
                 if (q) (void)hipFree(q);
             if (s->ev0) (void)hipEventDestroy(s->ev0);
             if (s->ev1) (void)hipEventDestroy(s->ev1);
+            order_free(s->ord);
             delete s;
         }
         free(d);
@@ -724,6 +857,37 @@ struct Host {
                       double z, double *d_input, const double *d_rtol, double *d_atol,
                       long long dims, const dengo_cell_outputs &out, hipStream_t stream,
                       unsigned long long *counter, bool events)
+    {
+        if (!o.order || dims <= 0)
+            return launch_cells(d, s, o, dtf, dt0, z, d_input, d_rtol, d_atol, dims, out, stream,
+                                counter, events);
+        // cells in hand-out order: gather, integrate, scatter back (the
+        // buffers were allocated and the order checked by the caller)
+        OrderBufs &b = s->ord;
+        if (b.cap < dims || b.n < N) return set_error("cell order buffers not prepared");
+        dengo_options oc = o;
+        oc.order = nullptr;
+        const dengo_cell_outputs pout = order_outputs(b, out);
+        const unsigned gr = (unsigned)((dims * N + 255) / 256), gc = (unsigned)((dims + 255) / 256);
+        hipLaunchKernelGGL((k_order_rows<N, true>), dim3(gr), dim3(256), 0, stream, o.order, dims,
+                           d_input, d_rtol, d_atol, b.in, b.rt, b.at);
+        DENGO_HIP_OK(hipGetLastError());
+        if (launch_cells(d, s, oc, dtf, dt0, z, b.in, b.rt, b.at, dims, pout, stream, counter,
+                         events))
+            return -1;
+        hipLaunchKernelGGL((k_order_rows<N, false>), dim3(gr), dim3(256), 0, stream, o.order, dims,
+                           b.in, b.at, nullptr, d_input, d_atol, nullptr);
+        DENGO_HIP_OK(hipGetLastError());
+        hipLaunchKernelGGL(k_order_outputs<0>, dim3(gc), dim3(256), 0, stream, o.order, dims, pout,
+                           out);
+        DENGO_HIP_OK(hipGetLastError());
+        return 0;
+    }
+
+    static int launch_cells(Data *d, DevState *s, const dengo_options &o, double dtf, double dt0,
+                            double z, double *d_input, const double *d_rtol, double *d_atol,
+                            long long dims, const dengo_cell_outputs &out, hipStream_t stream,
+                            unsigned long long *counter, bool events)
     {
         KParams<Net> p;
         p.input = d_input;
@@ -788,6 +952,9 @@ struct Host {
             if (!out.ttot) out.ttot = s->d_ttot;
             if (!out.dt) out.dt = s->d_dt;
         }
+        if (o.order && dims > 0 &&
+            (order_alloc(s->ord, dims, N) || order_check(s->ord, o.order, dims, st)))
+            return -1;
         d->current_z = z;
         if (launch(d, s, o, dtf, dt0, z, d_input, d_rtol, d_atol, dims, out, st, s->d_counter, true))
             return -1;
@@ -1025,6 +1192,17 @@ struct Host {
         return full_grid(s, oo, full);
     }
 
+    static int hook_prepare_order(void *data, const long long *order, long long n, void *stream)
+    {
+        Data *d = (Data *)data;
+        DevState *s = state(d);
+        if (ensure_device(d, s)) return -1;
+        if (!order || n <= 0) return 0;
+        return order_alloc(s->ord, n, N) || order_check(s->ord, order, n, (hipStream_t)stream)
+                   ? -1
+                   : 0;
+    }
+
     static int hook_launch(void *data, const LaunchArgs &a)
     {
         Data *d = (Data *)data;
@@ -1040,6 +1218,7 @@ struct Host {
         h.nspecies = N;
         h.isothermal = 0;
         h.prepare = hook_prepare;
+        h.prepare_order = hook_prepare_order;
         h.launch = hook_launch;
         return h;
     }

@@ -31,6 +31,7 @@ struct synthetic_data {
     double *d_dt = nullptr, *d_ttot = nullptr;
     int *d_status = nullptr, *d_nsub = nullptr, *d_nrej = nullptr, *d_nsweep = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    OrderBufs ord;                     // cell order check (the kernel reads the order itself)
 };
 
 namespace {
@@ -89,6 +90,7 @@ B::NetDev net_view(const synthetic_data *d)
     v.own_cnt_off = d->prog.own_cnt_off;
     v.diag_len = d->prog.diag_len;
     v.diag_off = d->prog.diag_off;
+    v.order = nullptr;
     return v;
 }
 
@@ -160,8 +162,10 @@ int launch(synthetic_data *d, const dengo_options &o, double dtf, double dt0, co
         hipLaunchKernelGGL(k_reset_counter<0>, dim3(1), dim3(1), 0, stream, counter);
         DENGO_HIP_OK(hipGetLastError());
     }
-    hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(B::NTHR), 0, stream, net_view(d), dtf, dt0,
-                       d_T, d_x, d_rtol, d_atol, dims, out, o.max_substeps, counter);
+    B::NetDev nv = net_view(d);
+    nv.order = o.order;
+    hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(B::NTHR), 0, stream, nv, dtf, dt0, d_T, d_x,
+                       d_rtol, d_atol, dims, out, o.max_substeps, counter);
     DENGO_HIP_OK(hipGetLastError());
     if (events) DENGO_HIP_OK(hipEventRecord(d->ev1, stream));
     return 0;
@@ -196,6 +200,14 @@ int hook_prepare(void *data, const dengo_options *opt, int *full)
     if (per_cu < 1) per_cu = 1;
     *full = per_cu * d->cu_count;
     return 0;
+}
+
+int hook_prepare_order(void *data, const long long *order, long long n, void *stream)
+{
+    synthetic_data *d = (synthetic_data *)data;
+    if (ensure_device(d)) return -1;
+    if (!order || n <= 0) return 0;
+    return order_alloc(d->ord, n, 0) || order_check(d->ord, order, n, (hipStream_t)stream) ? -1 : 0;
 }
 
 int hook_launch(void *data, const LaunchArgs &a)
@@ -253,6 +265,7 @@ void synthetic_free_data(synthetic_data *d)
             if (p) (void)hipFree(p);
         if (d->ev0) (void)hipEventDestroy(d->ev0);
         if (d->ev1) (void)hipEventDestroy(d->ev1);
+        order_free(d->ord);
     }
     delete d;
 }
@@ -277,6 +290,9 @@ int dengo_synthetic_evolve_device(synthetic_data *d, const dengo_options *opt, d
         if (!out.ttot) out.ttot = d->d_ttot;
         if (!out.dt) out.dt = d->d_dt;
     }
+    if (o.order && dims > 0 &&
+        (order_alloc(d->ord, dims, 0) || order_check(d->ord, o.order, dims, st)))
+        return -1;
     if (dims > 0 && launch(d, o, dtf, dt0, d_T, d_input, d_rtol, d_atol, dims, out, st,
                            d->d_counter, true))
         return -1;
@@ -397,6 +413,7 @@ NetHooks dengo::hooks_synthetic()
     h.nspecies = B::N;
     h.isothermal = 1;
     h.prepare = hook_prepare;
+    h.prepare_order = hook_prepare_order;
     h.launch = hook_launch;
     return h;
 }

@@ -28,7 +28,8 @@ class Options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("schedule", ctypes.c_int),
                 ("block", ctypes.c_int), ("grid", ctypes.c_int),
                 ("max_substeps", ctypes.c_int), ("input_is_number", ctypes.c_int),
-                ("reserved", ctypes.c_int * 6)]
+                ("reserved", ctypes.c_int * 2), ("order", ctypes.c_void_p),
+                ("reserved2", ctypes.c_int * 2)]
 
 
 class CellOutputs(ctypes.Structure):
@@ -224,7 +225,10 @@ def _p(a, t=_dp):
 
 
 def options(precision=FP64, schedule=PERSISTENT, grid=0, max_substeps=50000,
-            input_is_number=False):
+            input_is_number=False, order=None):
+    """dengo_options.  order: None, or a contiguous int64 CUDA tensor with the
+    hand-out order of the cells (see cost_order); the caller keeps it alive
+    until the launch has completed."""
     o = Options()
     load().dengo_default_options(ctypes.byref(o))
     o.precision = precision
@@ -232,7 +236,26 @@ def options(precision=FP64, schedule=PERSISTENT, grid=0, max_substeps=50000,
     o.grid = grid
     o.max_substeps = max_substeps
     o.input_is_number = int(bool(input_is_number))
+    if order is not None:
+        if not (getattr(order, "is_cuda", False) and order.dtype.itemsize == 8
+                and not order.dtype.is_floating_point and order.is_contiguous()):
+            raise ValueError("order must be a contiguous int64 CUDA tensor")
+        o.order = order.data_ptr()
     return o
+
+
+def _check_order(opt, n):
+    order = opt.get("order")
+    if order is not None and order.numel() < n:
+        raise ValueError("order holds %d entries for %d cells" % (order.numel(), n))
+
+
+def cost_order(nsweep):
+    """The hand-out order that puts the costly cells first: cell indices by
+    descending sweep count (e.g. the nsweep output of the previous call on the
+    same cells), as an int64 tensor on nsweep's device."""
+    import torch
+    return torch.argsort(nsweep.to(torch.int64), descending=True, stable=True).contiguous()
 
 
 def write_table_blob(path, block, T_bounds, z_bounds=(0.0, 0.0)):
@@ -288,6 +311,7 @@ class NetworkSolver(object):
         res = dict(status=np.zeros(n, np.int32), nsub=np.zeros(n, np.int32),
                    nrej=np.zeros(n, np.int32), nsweep=np.zeros(n, np.int32),
                    dt_cell=np.zeros(n), T=np.zeros(n))
+        _check_order(opt, n)
         o = options(**opt)
         dtc = ctypes.c_double(dt)
         ttot = self._fn("dengo_%s_evolve_host")(
@@ -316,6 +340,7 @@ class NetworkSolver(object):
             if v.numel() < n:
                 raise ValueError("output %s too small" % k)
             setattr(co, k, v.data_ptr())
+        _check_order(opt, n)
         o = options(**opt)
         tmin = ctypes.c_double(0.0)
         dmin = ctypes.c_double(0.0)
@@ -441,6 +466,7 @@ class SyntheticSolver(object):
         res = dict(status=np.zeros(n, np.int32), nsub=np.zeros(n, np.int32),
                    nrej=np.zeros(n, np.int32), nsweep=np.zeros(n, np.int32),
                    dt_cell=np.zeros(n))
+        _check_order(opt, n)
         o = options(**opt)
         o.block = 0
         dtc = ctypes.c_double(dt)
@@ -470,6 +496,7 @@ class SyntheticSolver(object):
             if v.numel() < n:
                 raise ValueError("output %s too small" % k)
             setattr(co, k, v.data_ptr())
+        _check_order(opt, n)
         o = options(**opt)
         o.block = 0
         tmin = ctypes.c_double(0.0)
@@ -627,9 +654,10 @@ class CapturedEvolve(object):
         co = CellOutputs()
         for k, v in (outputs or {}).items():
             setattr(co, k, v.data_ptr())
+        _check_order(opt, x.shape[0])
         o = options(**opt)
         o.block = 0
-        self._keep = (x, rtol, atol, T, outputs)
+        self._keep = (x, rtol, atol, T, outputs, opt.get("order"))
         self.g = load().dengo_graph_create(
             net, solver.data, ctypes.byref(o), dtf, dt0, z,
             None if T is None else ctypes.c_void_p(T.data_ptr()), ctypes.c_void_p(x.data_ptr()),

@@ -34,7 +34,19 @@ typedef struct dengo_options {
     int grid;             /* blocks for the persistent schedule (0: fill GPU) */
     int max_substeps;     /* accepted-substep cap, reference: 50000 */
     int input_is_number;  /* 1: species given as number densities */
-    int reserved[6];
+    int reserved[2];
+    const long long *order;  /* NULL, or device memory holding the order in
+                             which cells are handed out: cell order[0] first,
+                             then order[1], ... (a permutation of 0..dims-1).
+                             Handing out the costly cells first shortens the
+                             tail of a persistent launch; a previous call's
+                             per-cell sweep counts, sorted descending, are a
+                             good order.  Results per cell do not depend on
+                             it.  The evolve calls check that it is a
+                             permutation (one pass, one synchronisation) and
+                             fail otherwise; a captured graph checks it once,
+                             at creation. */
+    int reserved2[2];
 } dengo_options;
 
 /* Per-cell device outputs; any pointer may be NULL. */
