@@ -59,8 +59,8 @@ def log(msg):
 
 def serving_leg(args, prim):
     """Poisson arrivals of patches (edge 8-64 cells, primordial:oxygen:synthetic
-    = 6:3:1) through the patch server, mixed precision; returns the latency
-    and throughput summary."""
+    = 6:3:1) through the patch server, precision per network; returns the
+    latency and throughput summary."""
     from dengo_amd import runtime, serving
     from dengo_amd.synthetic import SyntheticNetwork
     oxy = runtime.OxygenSolver()
@@ -74,11 +74,31 @@ def serving_leg(args, prim):
         p, kp = serving.make_patch(k, n, s, synthetic_net=net)
         patches.append(p)
         keep.append(kp)
-    # mixed precision where it holds the reference's own noise floor (the
-    # oxygen network); the structure-exploiting FP64 sweep for the primordial
-    # network, whose capped cells the mixed sweep does not reproduce, and for
-    # the dense network, which has no FP32 factorization
-    if args.serve_precision == "mixed":
+    # precision per network: "auto" lets the autotuner pick, on a sample of
+    # cells of the patches' distributions, the fastest sweep that meets the
+    # 1e-6 bar against the faithful FP64 sweep (the dense network has no FP32
+    # factorization: FP64-fast); "parity" is the choice written out (mixed for
+    # oxygen only); "mixed" uses mixed wherever it exists
+    tuned = None
+    if args.serve_precision == "auto":
+        import torch
+        from dengo_amd import autotune
+        from dengo_amd.ics import oxygen_ics, primordial_ics, tolerances
+        dev = torch.device("cuda:0")
+        tuned = {}
+        for k, solver, ics in ((runtime.NET_PRIMORDIAL, prim, primordial_ics),
+                               (runtime.NET_OXYGEN, oxy, oxygen_ics)):
+            # a sample of a typical patch's size: the ranking of the sweeps
+            # depends on load (a small sample is timed by its longest cell)
+            xs, _, _ = ics(32768, seed=args.seed + 17)
+            rt, at = tolerances(xs)
+            tuned[k] = autotune.tune(solver, torch.from_numpy(xs).to(dev),
+                                     torch.from_numpy(rt).to(dev), torch.from_numpy(at).to(dev),
+                                     1e12, sample=32768, log=log)
+        policy = {runtime.NET_PRIMORDIAL: tuned[runtime.NET_PRIMORDIAL]["options"]["precision"],
+                  runtime.NET_OXYGEN: tuned[runtime.NET_OXYGEN]["options"]["precision"],
+                  runtime.NET_SYNTHETIC: runtime.FP64_FAST}
+    elif args.serve_precision == "mixed":
         policy = {runtime.NET_PRIMORDIAL: runtime.MIXED, runtime.NET_OXYGEN: runtime.MIXED,
                   runtime.NET_SYNTHETIC: runtime.FP64_FAST}
     else:
@@ -97,6 +117,10 @@ def serving_leg(args, prim):
                 "precision": {serving.NET_NAMES[k]: {runtime.FP64: "fp64", runtime.MIXED: "mixed",
                                                       runtime.FP64_FAST: "fp64-fast"}[v]
                               for k, v in policy.items()},
+                "precision_choice": args.serve_precision,
+                "autotune": None if tuned is None else {
+                    serving.NET_NAMES[k]: {"precision_trials": v["precision_trials"],
+                                           "sample": v["sample"]} for k, v in tuned.items()},
                 "arena_peak_bytes": stats["arena_peak"], "arena_bytes": stats["arena_bytes"],
                 "latency": "submit (host arrays in pinned memory) to copy-back complete"})
     log("serving: p50 %.1f ms, p99 %.1f ms, %.0f cell-solves/s" % (
@@ -183,9 +207,10 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=15.0,
                     help="patch server: arrival window (s); 0 skips the serving leg")
     ap.add_argument("--serve-slots", type=int, default=4)
-    ap.add_argument("--serve-precision", default="parity", choices=["parity", "mixed"],
-                    help="parity: mixed sweep where it holds the reference's noise floor "
-                         "(oxygen), FP64-fast elsewhere; mixed: mixed wherever it exists")
+    ap.add_argument("--serve-precision", default="auto", choices=["auto", "parity", "mixed"],
+                    help="auto: per network, the autotuner's fastest sweep that meets the 1e-6 "
+                         "bar on a sample; parity: mixed for oxygen, FP64-fast elsewhere; "
+                         "mixed: mixed wherever it exists")
     ap.add_argument("--serve-share", type=int, default=0,
                     help="patch server: grid share (0: the number of slots)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,

@@ -1,0 +1,137 @@
+"""Autotuner: per network, the fastest launch that meets the accuracy bar.
+
+The tuner times candidate launches on a sample of the caller's own cells (a
+few thousand, device-resident) and returns the options to use for the full
+workload:
+
+1. precision: the reference-faithful FP64 run of the sample is the yardstick;
+   each faster variant (mixed, then FP64-fast) is accepted only when at least
+   `min_agree` of the sample's cells end in the same status and, among the
+   cells both runs integrate to dtf, at least `min_agree` agree to `rtol_bar`
+   (maximum relative error over species and energy, the 1e-6 bar of the
+   reference comparison).  The fastest accepted variant wins; FP64 always
+   qualifies.
+2. schedule: persistent (full grid, half grid) or one cell per lane, timed at
+   the chosen precision; the fastest wins.
+
+Timings are kernel times from HIP events on the launch stream.  The sample
+should be as large as the batches the choice is for: the variants' ranking
+depends on load, and a small sample is timed by its longest cell.  Results can be
+cached in a JSON file keyed by device, network, unknowns and sample size, so a
+simulation tunes once and reuses the choice every step.
+"""
+import json
+import os
+
+import numpy as np
+
+from . import runtime
+
+PRECISION_NAMES = {runtime.FP64: "fp64", runtime.MIXED: "mixed", runtime.FP64_FAST: "fp64-fast"}
+
+
+def _name(solver):
+    if isinstance(solver, runtime.SyntheticSolver):
+        return "synthetic%d" % solver.N
+    return getattr(solver, "name", type(solver).__name__)
+
+
+def _precisions(solver):
+    # the dense network has no FP32 factorization
+    if isinstance(solver, runtime.SyntheticSolver):
+        return [runtime.FP64_FAST]
+    return [runtime.MIXED, runtime.FP64_FAST]
+
+
+def _run(solver, x, rt, at, dtf, T, **opt):
+    import torch
+    xs, ats = x.clone(), at.clone()
+    n = x.shape[0]
+    st = torch.zeros(n, dtype=torch.int32, device=x.device)
+    if T is None:
+        solver.evolve_device(xs, rt, ats, dtf, outputs={"status": st}, **opt)
+    else:
+        solver.evolve_device(xs, T, rt, ats, dtf, outputs={"status": st}, **opt)
+    torch.cuda.synchronize()
+    return runtime.load().dengo_last_kernel_ms(), xs, st
+
+
+def agreement(x_ref, st_ref, x, st, rtol_bar):
+    """(status agreement, fraction of cells within rtol_bar) of a run against
+    the reference run; the second over cells both runs integrate to dtf."""
+    same_status = float((st == st_ref).double().mean())
+    done = (st_ref == 0) & (st == 0)
+    if int(done.sum()) == 0:
+        return same_status, 1.0
+    rel = ((x - x_ref).abs() / x_ref.abs().clamp_min(1e-300))[done].max(dim=1).values
+    return same_status, float((rel <= rtol_bar).double().mean())
+
+
+def tune(solver, x, rtol, atol, dtf, T=None, sample=4096, rtol_bar=1e-6, min_agree=0.99,
+         repeats=1, cache=None, log=None):
+    """Options (precision, schedule, grid) for solver on cells like x.
+
+    x, rtol, atol (and T for the dense network) are device tensors [n, N]; the
+    first `sample` cells are used.  Returns a dict with "options" (keyword
+    arguments for evolve_device) and the measurements behind the choice."""
+    import torch
+    n = min(sample, x.shape[0])
+    dev = torch.cuda.get_device_properties(x.device)
+    key = "%s|%s|%d|%d|%g|%g" % (dev.name, _name(solver), solver.N, n, rtol_bar, min_agree)
+    if cache and os.path.exists(cache):
+        with open(cache) as f:
+            saved = json.load(f)
+        if key in saved:
+            return saved[key]
+    xs, rt, at = x[:n].contiguous(), rtol[:n].contiguous(), atol[:n].contiguous()
+    Ts = None if T is None else T[:n].contiguous()
+
+    def timed(**opt):
+        ms = []
+        for _ in range(repeats):
+            t, xo, so = _run(solver, xs, rt, at, dtf, Ts, **opt)
+            ms.append(t)
+        return float(np.median(ms)), xo, so
+
+    base_ms, x_ref, st_ref = timed(precision=runtime.FP64)
+    trials = {"fp64": {"kernel_ms": base_ms, "status_agreement": 1.0, "within_bar": 1.0,
+                       "accepted": True}}
+    best_prec, best_ms = runtime.FP64, base_ms
+    for prec in _precisions(solver):
+        ms, xo, so = timed(precision=prec)
+        sa, wb = agreement(x_ref, st_ref, xo, so, rtol_bar)
+        ok = sa >= min_agree and wb >= min_agree
+        trials[PRECISION_NAMES[prec]] = {"kernel_ms": ms, "status_agreement": sa, "within_bar": wb,
+                                         "accepted": ok}
+        if log:
+            log("autotune %s %s: %.1f ms, status %.4f, within %.4f%s" % (
+                _name(solver), PRECISION_NAMES[prec], ms, sa, wb, "" if ok else " (rejected)"))
+        if ok and ms < best_ms:
+            best_prec, best_ms = prec, ms
+    # resident workgroups that fill the GPU: one dense cell per CU, one
+    # lane-per-cell wave per SIMD
+    full = dev.multi_processor_count * (1 if isinstance(solver, runtime.SyntheticSolver) else 4)
+    sched = {}
+    cands = [("persistent", dict(schedule=runtime.PERSISTENT, grid=0)),
+             ("persistent_half_grid", dict(schedule=runtime.PERSISTENT, grid=max(1, full // 2))),
+             ("per_cell", dict(schedule=runtime.PER_CELL))]
+    best = None
+    for name, opt in cands:
+        ms, _, _ = timed(precision=best_prec, **opt)
+        sched[name] = ms
+        if best is None or ms < best[1]:
+            best = (name, ms, opt)
+    result = {"key": key, "options": dict(precision=best_prec, **best[2]),
+              "precision": PRECISION_NAMES[best_prec], "schedule": best[0],
+              "precision_trials": trials, "schedule_ms": sched, "sample": n}
+    if cache:
+        saved = {}
+        if os.path.exists(cache):
+            with open(cache) as f:
+                saved = json.load(f)
+        saved[key] = result
+        d = os.path.dirname(os.path.abspath(cache))
+        os.makedirs(d, exist_ok=True)
+        with open(cache, "w") as f:
+            json.dump(saved, f, indent=1)
+    return result

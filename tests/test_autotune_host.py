@@ -1,0 +1,18 @@
+"""The autotuner's accuracy measure on the CPU (torch tensors)."""
+import torch
+
+from dengo_amd.autotune import agreement
+
+
+def test_agreement_counts_status_and_relative_error():
+    ref = torch.ones(4, 3, dtype=torch.float64)
+    st_ref = torch.tensor([0, 0, 0, 1], dtype=torch.int32)
+    x = ref.clone()
+    x[1, 2] = 1.0 + 2e-6          # outside a 1e-6 bar
+    x[2, 0] = 1.0 + 5e-7          # inside
+    st = torch.tensor([0, 0, 0, 2], dtype=torch.int32)
+    sa, wb = agreement(ref, st_ref, x, st, 1e-6)
+    assert sa == 0.75             # the last cell's status differs
+    assert abs(wb - 2.0 / 3.0) < 1e-12   # over the three cells both runs finish
+    # nothing finished in both runs: the bar holds vacuously
+    assert agreement(ref, st_ref + 1, x, st_ref + 1, 1e-6) == (1.0, 1.0)
