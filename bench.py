@@ -104,8 +104,12 @@ def serving_leg(args, prim):
     else:
         policy = {runtime.NET_PRIMORDIAL: runtime.FP64_FAST, runtime.NET_OXYGEN: runtime.MIXED,
                   runtime.NET_SYNTHETIC: runtime.FP64_FAST}
+    shares = {}
+    if args.serve_dense_share > 0:
+        shares[runtime.NET_SYNTHETIC] = args.serve_dense_share
     srv = runtime.PatchServer(prim, oxy, syn, precision=runtime.MIXED, slots=args.serve_slots,
-                              grid_share=args.serve_share, net_precision=policy)
+                              grid_share=args.serve_share, net_precision=policy,
+                              net_grid_share=shares)
     try:
         recs, t0 = serving.replay(srv, trace, patches, 1e12, timeout_s=900.0)
         stats = srv.stats()
@@ -114,6 +118,7 @@ def serving_leg(args, prim):
     out = serving.summarize(trace, recs, t0)
     out.update({"rate_per_s": args.serve_rate, "window_s": args.serve_seconds,
                 "slots": args.serve_slots, "grid_share": args.serve_share or args.serve_slots,
+                "grid_share_dense": args.serve_dense_share or args.serve_share or args.serve_slots,
                 "precision": {serving.NET_NAMES[k]: {runtime.FP64: "fp64", runtime.MIXED: "mixed",
                                                       runtime.FP64_FAST: "fp64-fast"}[v]
                               for k, v in policy.items()},
@@ -213,6 +218,9 @@ def main():
                          "mixed: mixed wherever it exists")
     ap.add_argument("--serve-share", type=int, default=0,
                     help="patch server: grid share (0: the number of slots)")
+    ap.add_argument("--serve-dense-share", type=int, default=2,
+                    help="patch server: grid share of the dense network's patches, whose capped "
+                         "cells run for tens of seconds (0: --serve-share)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (counter calibration)")
     args = ap.parse_args()

@@ -302,6 +302,7 @@ void dengo_server_default_config(dengo_server_config *cfg)
     cfg->grid_share = 0;
     cfg->max_bypass = 8;
     for (int k = 0; k < 3; ++k) cfg->precision[k] = -1;
+    for (int k = 0; k < 3; ++k) cfg->grid_shares[k] = 0;
 }
 
 dengo_server *dengo_server_create(const dengo_server_config *cfg_in, void *primordial,
@@ -324,6 +325,11 @@ dengo_server *dengo_server_create(const dengo_server_config *cfg_in, void *primo
         set_error("unknown precision %d", precision);
         return nullptr;
     }
+    for (int k = 0; k < kNets; ++k)
+        if (cfg.grid_shares[k] < 0 || cfg.grid_shares[k] > 1024) {
+            set_error("grid_shares[%d] must be in [0, 1024]", k);
+            return nullptr;
+        }
     for (int k = 0; k < kNets; ++k)
         if (cfg.precision[k] > DENGO_FP64_FAST || cfg.precision[k] < -1) {
             set_error("unknown precision %d for network %d", cfg.precision[k], k);
@@ -358,7 +364,8 @@ dengo_server *dengo_server_create(const dengo_server_config *cfg_in, void *primo
         o.block = 0;
         int full = 0;
         if (s->hooks[k].prepare(s->data[k], &o, &full)) return fail("prepare", hipSuccess);
-        s->grid_cap[k] = full / cfg.grid_share > 0 ? full / cfg.grid_share : 1;
+        const int share = cfg.grid_shares[k] > 0 ? cfg.grid_shares[k] : cfg.grid_share;
+        s->grid_cap[k] = full / share > 0 ? full / share : 1;
     }
     if ((e = hipMalloc(&s->arena_base, cfg.hbm_budget)) != hipSuccess)
         return fail("arena hipMalloc", e);

@@ -41,7 +41,8 @@ class CellOutputs(ctypes.Structure):
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_size_t), ("slots", ctypes.c_int),
                 ("grid_share", ctypes.c_int), ("max_bypass", ctypes.c_int),
-                ("precision", ctypes.c_int * 3), ("reserved", ctypes.c_int * 2)]
+                ("precision", ctypes.c_int * 3), ("grid_shares", ctypes.c_int * 3),
+                ("reserved", ctypes.c_int)]
 
 
 class PatchResult(ctypes.Structure):
@@ -559,8 +560,10 @@ class PatchServer(object):
     reference)."""
 
     def __init__(self, primordial=None, oxygen=None, synthetic=None, precision=MIXED,
-                 hbm_budget=8 << 30, slots=4, grid_share=0, max_bypass=8, net_precision=None):
-        """net_precision: optional {network: precision} overriding precision."""
+                 hbm_budget=8 << 30, slots=4, grid_share=0, max_bypass=8, net_precision=None,
+                 net_grid_share=None):
+        """net_precision: optional {network: precision} overriding precision;
+        net_grid_share: optional {network: grid share} overriding grid_share."""
         lib = load()
         cfg = ServerConfig()
         lib.dengo_server_default_config(ctypes.byref(cfg))
@@ -570,6 +573,8 @@ class PatchServer(object):
         cfg.max_bypass = max_bypass
         for k, v in (net_precision or {}).items():
             cfg.precision[k] = v
+        for k, v in (net_grid_share or {}).items():
+            cfg.grid_shares[k] = v
         self._solvers = (primordial, oxygen, synthetic)
         ptr = [None if x is None else x.data for x in self._solvers]
         self.srv = lib.dengo_server_create(ctypes.byref(cfg), ptr[0], ptr[1], ptr[2], precision)

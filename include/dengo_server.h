@@ -53,7 +53,10 @@ typedef struct dengo_server_config {
     int precision[3];     /* per network (enum dengo_network): the Newton
                              sweep's precision, or -1 for the one given to
                              dengo_server_create */
-    int reserved[2];
+    int grid_shares[3];   /* per network: grid_share for its patches, or 0 for
+                             grid_share (a network whose patches are long may
+                             take a larger share of the GPU) */
+    int reserved;
 } dengo_server_config;
 
 /* Completion record of one request. */
@@ -81,8 +84,8 @@ typedef struct dengo_server_stats {
     double first_submit, last_done;      /* server steady clock, s */
 } dengo_server_stats;
 
-/* Defaults: 8 GiB arena, 4 slots, grid_share = slots, max_bypass 8, every
- * network at the precision given to dengo_server_create. */
+/* Defaults: 8 GiB arena, 4 slots, grid_share = slots for every network,
+ * max_bypass 8, every network at the precision given to dengo_server_create. */
 void dengo_server_default_config(dengo_server_config *cfg);
 
 /* Creates a server on the current device for the given data structs (tables

@@ -30,6 +30,7 @@ def test_server_config_defaults():
     lib.dengo_server_default_config(cfg)
     assert cfg.hbm_budget == 8 << 30 and cfg.slots == 4 and cfg.max_bypass == 8
     assert cfg.grid_share == 0 and list(cfg.precision) == [-1, -1, -1]
+    assert list(cfg.grid_shares) == [0, 0, 0]
 
 
 def test_server_create_rejects_bad_arguments():
@@ -45,3 +46,7 @@ def test_server_create_rejects_bad_arguments():
     cfg.precision[1] = 7
     assert not lib.dengo_server_create(cfg, None, None, None, runtime.MIXED)
     assert "unknown precision 7 for network 1" in runtime.last_error()
+    lib.dengo_server_default_config(cfg)
+    cfg.grid_shares[2] = -1
+    assert not lib.dengo_server_create(cfg, None, None, None, runtime.MIXED)
+    assert "grid_shares[2]" in runtime.last_error()
