@@ -70,6 +70,7 @@ template <int M, const LUPlan<M> &P, class T>
 DENGO_D void block_lu_factor(T (&A)[M][M], T (&rp)[M], int (&piv)[M])
 {
 #pragma clang fp contract(fast)
+    constexpr bool kSkip = sizeof(T) == sizeof(float);
     static_for<0, M>([&](auto kk) {
         constexpr int k = CV(kk);
         piv[k] = k;
@@ -88,18 +89,22 @@ DENGO_D void block_lu_factor(T (&A)[M][M], T (&rp)[M], int (&piv)[M])
             });
             piv[k] = p;
             if (__any(p != k)) {
+                // FP32 factor: a candidate row's exchange (a select per entry
+                // of both rows) runs only if some lane of the wave chose it
                 static_for<k + 1, M>([&](auto ii) {
                     constexpr int i = CV(ii);
                     if constexpr (P.cand[k][i]) {
                         const bool sw = (p == i);
-                        static_for<k, M>([&](auto jj) {
-                            constexpr int j = CV(jj);
-                            if constexpr (P.S[k][k][j] || P.S[k][i][j]) {
-                                const T a = A[k][j], c = A[i][j];
-                                A[k][j] = sw ? c : a;
-                                A[i][j] = sw ? a : c;
-                            }
-                        });
+                        if (!kSkip || __any(sw)) {
+                            static_for<k, M>([&](auto jj) {
+                                constexpr int j = CV(jj);
+                                if constexpr (P.S[k][k][j] || P.S[k][i][j]) {
+                                    const T a = A[k][j], c = A[i][j];
+                                    A[k][j] = sw ? c : a;
+                                    A[i][j] = sw ? a : c;
+                                }
+                            });
+                        }
                     }
                 });
             }
@@ -127,6 +132,7 @@ DENGO_D void block_lu_apply(const T (&A)[M][M], const T (&rp)[M], const int (&pi
                             T (&x)[M])
 {
 #pragma clang fp contract(fast)
+    constexpr bool kSkip = sizeof(T) == sizeof(float);
     static_for<0, M - 1>([&](auto kk) {
         constexpr int k = CV(kk);
         if (__any(piv[k] != k)) {
@@ -134,9 +140,11 @@ DENGO_D void block_lu_apply(const T (&A)[M][M], const T (&rp)[M], const int (&pi
                 constexpr int i = CV(ii);
                 if constexpr (P.cand[k][i]) {
                     const bool sw = (piv[k] == i);
-                    const T xa = x[k], xc = x[i];
-                    x[k] = sw ? xc : xa;
-                    x[i] = sw ? xa : xc;
+                    if (!kSkip || __any(sw)) {
+                        const T xa = x[k], xc = x[i];
+                        x[k] = sw ? xc : xa;
+                        x[i] = sw ? xa : xc;
+                    }
                 }
             });
         }
