@@ -202,6 +202,8 @@ def main():
                     help="comma-separated extra variants timed once (empty: none)")
     ap.add_argument("--no-percell", action="store_true", help="skip the plain per-cell kernel")
     ap.add_argument("--cpu-cells", type=int, default=4096)
+    ap.add_argument("--cpu-repeats", type=int, default=3,
+                    help="CPU baseline runs (the median is reported)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--oxygen-cells", type=int, default=1 << 20,
                     help="oxygen ion-by-ion leg (0 skips it)")
@@ -393,13 +395,31 @@ def main():
         import oracle
         m = min(args.cpu_cells, n)
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
-        log("cpu baseline: oracle on %d cells, %d threads" % (m, threads))
+        log("cpu baseline: oracle on %d cells, %d threads, %d runs" % (m, threads,
+                                                                      args.cpu_repeats))
+        els = []
+        for _ in range(max(1, args.cpu_repeats)):
+            t0 = time.perf_counter()
+            oracle.prim_evolve(inp[:m], rtol[:m], atol[:m], tab, 1e12, nthreads=threads)
+            els.append(time.perf_counter() - t0)
+        el = float(np.median(els))
+        m1 = min(128, m)
         t0 = time.perf_counter()
-        oracle.prim_evolve(inp[:m], rtol[:m], atol[:m], tab, 1e12, nthreads=threads)
-        el = time.perf_counter() - t0
+        oracle.prim_evolve(inp[:m1], rtol[:m1], atol[:m1], tab, 1e12, nthreads=1)
+        el1 = time.perf_counter() - t0
+        model = "unknown"
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+        except (OSError, StopIteration):
+            pass
         cpu = {"value": m / el, "unit": "cell-solves/s", "cores": threads, "kind": "port",
                "sample": "first %d of the %d cells (same seeded inputs), CPU FP64 oracle, "
-                         "OpenMP dynamic schedule, %.1f s" % (m, n, el)}
+                         "OpenMP dynamic schedule, median of %d runs of %.1f s" % (
+                             m, n, len(els), el),
+               "cpu_model": model,
+               "single_core": {"value": m1 / el1, "unit": "cell-solves/s", "cores": 1,
+                               "sample": "first %d cells, %.1f s" % (m1, el1)}}
 
     out = {
         "metric": "cell-solves/s (primordial BE solve, dtf=1e12 s)",
