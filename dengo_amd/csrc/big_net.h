@@ -22,9 +22,11 @@
 // wave owning the panel factors it alone - pivot search by wave reductions
 // (DPP), pivot row by lane reads, no workgroup barrier per step - and writes
 // the 16 multiplier columns to LDS; then every wave applies the panel's 16
-// steps, in order, to its own columns right of the panel.  Every matrix
-// element receives exactly the updates of the unblocked algorithm in the same
-// order, so the faithful mode stays bit-identical to the reference.
+// steps to its own columns right of the panel.  In the faithful mode every
+// matrix element receives exactly the updates of the unblocked algorithm in
+// the same order, so it stays bit-identical to the reference; the
+// Gauss-Jordan mode applies a panel as one rank-16 update on the matrix cores
+// (apply_panel_gj).
 //
 // Partial pivoting is virtual: rows never move.  pos[row] is the row's current
 // position in the reference's swapped order, so the pivot search reproduces
@@ -63,6 +65,12 @@ constexpr int BUF_SZ = MAX_RXN + 32;
 static_assert(CHUNK_SZ <= BUF_SZ && TILE * N <= BUF_SZ, "shared buffer size");
 
 enum { kLU = 0, kGJ = 1 };
+
+// The Gauss-Jordan mode's trailing update: 1 on the matrix cores
+// (apply_panel_gj), 0 step by step on the vector ALU (apply_panel).
+#ifndef DENGO_TRAILING_MFMA
+#define DENGO_TRAILING_MFMA 1
+#endif
 
 // RHS operation: bits 0-7 species (N: padding), bits 8-10 change + 4,
 // bits 11-31 reaction.
@@ -600,6 +608,92 @@ __device__ __forceinline__ void apply_panel(double (&M)[TCW][RPL][TILE], Shared 
     }
 }
 
+// Orders this wave's LDS writes before its later LDS reads by other lanes
+// (one wave's LDS operations complete in issue order).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+// Gauss-Jordan application of panel P to tile column slot TC as one rank-16
+// update on the matrix cores.  Step s of the panel subtracts l_s[r] u_s from
+// every row r, where u_s is pivot row p_s at step s and l_s[p_s] = 0, so
+// after the 16 steps A = A0 - L U with L[r][s] = l_s[r] (S.buf) and U the
+// pivot rows at their own steps:
+//   U_s = A0[p_s] - sum_{s' < s} L[p_s][s'] U_s'   (forward substitution, the
+//         same FMA sequence the step-by-step update gives those rows),
+// found by lane c % 16 for column c of the tile.  L U is formed transposed,
+// D_b = U^T L_b^T for each 16-row block b by v_mfma_f64_16x16x4f64 (lane
+// j + 16 g holds D_b rows 4r + g, r = 0..3, column j: row 16 b + j of L U,
+// columns 4r + g), and passes through the wave's LDS buffer W to the lane
+// that holds the matrix row.  Differs from the step-by-step update by rounding
+// only.
+constexpr int WLD = 20;                 // W row stride (doubles; 16-byte aligned rows)
+constexpr int W_SZ = 64 * WLD;          // one half of a tile column
+
+template <int TC>
+__device__ __forceinline__ void apply_panel_gj(double (&M)[TCW][RPL][TILE], Shared &S, const Geo &G,
+                                               int P, double *W)
+{
+    const int k0 = TILE * P;
+    // pivot row p_s (pos[p_s] = k0 + s after the panel) publishes its values
+#pragma unroll
+    for (int h = 0; h < RPL; ++h) {
+        const unsigned s = (unsigned)(S.pos[G.row(h)] - k0);
+        if (s < (unsigned)TILE) {
+#pragma unroll
+            for (int c = 0; c < TILE; ++c) W[s * TILE + c] = M[TC][h][c];
+        }
+    }
+    wave_lds_sync();
+    const int c = G.lane & 15, g = G.lane >> 4;
+    double U[TILE];
+#pragma unroll
+    for (int s = 0; s < TILE; ++s) U[s] = W[s * TILE + c];
+    int pr[TILE];
+#pragma unroll
+    for (int s = 0; s < TILE; ++s) pr[s] = __builtin_amdgcn_readfirstlane(S.prow[k0 + s]);
+#pragma unroll
+    for (int s = 0; s < TILE; ++s)
+#pragma unroll
+        for (int t = s + 1; t < TILE; ++t) U[t] = fma(-S.buf[s * N + pr[t]], U[s], U[t]);
+    // A operand of chunk kc: U[4 kc + g][c], through LDS (the four row groups
+    // hold the same U and write the same values)
+    wave_lds_sync();
+#pragma unroll
+    for (int s = 0; s < TILE; ++s) W[s * TILE + c] = U[s];
+    wave_lds_sync();
+    double ua[4];
+#pragma unroll
+    for (int kc = 0; kc < 4; ++kc) ua[kc] = W[(4 * kc + g) * TILE + c];
+    wave_lds_sync();
+#pragma unroll
+    for (int h = 0; h < RPL; ++h) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            double4_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int kc = 0; kc < 4; ++kc) {
+                // B operand: L[16 b + c][4 kc + g], block b = 4 h + q
+                const double lv = S.buf[(4 * kc + g) * N + TILE * (4 * h + q) + c];
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ua[kc], lv, acc, 0, 0, 0);
+            }
+            double *w = W + (TILE * q + c) * WLD + g;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) w[4 * r] = acc[r];
+        }
+        wave_lds_sync();
+        const double *w = W + G.lane * WLD;
+#pragma unroll
+        for (int cc = 0; cc < TILE; ++cc) M[TC][h][cc] = M[TC][h][cc] - w[cc];
+        wave_lds_sync();
+    }
+}
+
 template <int MODE>
 __device__ __forceinline__ void dense_solve(double (&M)[TCW][RPL][TILE], Shared &S, const Geo &G,
                                             double *Ul)
@@ -611,7 +705,11 @@ __device__ __forceinline__ void dense_solve(double (&M)[TCW][RPL][TILE], Shared 
         }
         __syncthreads();
         static_for<0, TCW>([&](auto TC) {
-            if (G.w + NWAVE * CV(TC) > P) apply_panel<MODE, CV(TC)>(M, S, G, P, Ul);
+            if (G.w + NWAVE * CV(TC) > P) {
+                if constexpr (MODE == kGJ && DENGO_TRAILING_MFMA)
+                    apply_panel_gj<CV(TC)>(M, S, G, P, Ul + G.w * W_SZ);
+                else apply_panel<MODE, CV(TC)>(M, S, G, P, Ul);
+            }
         });
         __syncthreads();
     }
@@ -758,7 +856,9 @@ __global__ __launch_bounds__(NTHR) void k_big(NetDev net, double dtf, double dt_
                                               unsigned long long *counter)
 {
     __shared__ Shared S;
-    __shared__ double Ul[MODE == kLU ? U_PACKED : 1];
+    // the faithful mode's packed U rows; the Gauss-Jordan mode's per-wave
+    // buffers of the trailing update
+    __shared__ double Ul[MODE == kLU ? U_PACKED : (DENGO_TRAILING_MFMA ? NWAVE * W_SZ : 1)];
     const Geo G;
     const int t = G.t;
     double M[TCW][RPL][TILE];

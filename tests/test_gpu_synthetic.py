@@ -50,17 +50,25 @@ def test_synthetic_fp64_bitwise(syn, schedule):
 
 
 def test_synthetic_fp64_fast(syn):
-    """Gauss-Jordan with FMA: same status everywhere; on cells that take the
-    oracle's substep path (>= 90%) every species above 1e-10 of its cell's
-    density agrees to 1e-6, and every species stays within the solver's own
-    tolerance (|d| <= atol + rtol |x|)."""
+    """Gauss-Jordan with FMA and the matrix-core trailing update: same status
+    everywhere; on the cells that reach dtf along the oracle's substep path
+    (>= 90%) every species above 1e-10 of its cell's density agrees to 1e-6,
+    and every species stays within the solver's own tolerance
+    (|d| <= atol + rtol |x|).  A cell that runs to the substep cap is chaotic:
+    its state after 50,000 substeps is decided by rounding (0.5 relative in
+    the step-by-step and the blocked update alike), so it is held to its
+    status and to the particle count every reaction conserves."""
     s, _ = syn
     g = np.load(GOLD)
     r = s.evolve(g["input"], g["T"], g["rtol"], g["atol"], float(g["dtf"]),
                  precision=runtime.FP64_FAST)
     ref = g["state"]
     assert np.array_equal(r["status"], g["status"])
-    same = (r["nsub"] == g["nsub"]) & (r["nrej"] == g["nrej"])
+    capped = g["status"] == 2
+    assert capped.sum() == 1
+    tot0, tot1 = g["input"].sum(axis=1), r["state"].sum(axis=1)
+    assert np.all(np.abs(tot1 - tot0)[capped] <= 1e-9 * tot0[capped])
+    same = (r["nsub"] == g["nsub"]) & (r["nrej"] == g["nrej"]) & ~capped
     assert same.mean() >= 0.9
     frac = ref / ref.sum(axis=1, keepdims=True)
     rel = np.abs(r["state"] - ref) / np.abs(ref)
