@@ -65,12 +65,13 @@ DENGO_D float fast_rcp(float x) { return 1.0f / x; }
 
 // Factor A in place (partial pivoting: largest |a| among the structural
 // candidates, first on ties); multipliers are stored below the diagonal,
-// reciprocal pivots in rp, the pivot row of each step in piv.
-template <int M, const LUPlan<M> &P, class T>
+// reciprocal pivots in rp, the pivot row of each step in piv.  kSkip: skip
+// each candidate row's exchange that no lane of the wave chose (the same
+// pivots; pays where it does not move the caller's registers to scratch).
+template <int M, const LUPlan<M> &P, bool kSkip = false, class T>
 DENGO_D void block_lu_factor(T (&A)[M][M], T (&rp)[M], int (&piv)[M])
 {
 #pragma clang fp contract(fast)
-    constexpr bool kSkip = sizeof(T) == sizeof(float);
     static_for<0, M>([&](auto kk) {
         constexpr int k = CV(kk);
         piv[k] = k;
@@ -89,8 +90,8 @@ DENGO_D void block_lu_factor(T (&A)[M][M], T (&rp)[M], int (&piv)[M])
             });
             piv[k] = p;
             if (__any(p != k)) {
-                // FP32 factor: a candidate row's exchange (a select per entry
-                // of both rows) runs only if some lane of the wave chose it
+                // with kSkip a candidate row's exchange (a select per entry of
+                // both rows) runs only if some lane of the wave chose it
                 static_for<k + 1, M>([&](auto ii) {
                     constexpr int i = CV(ii);
                     if constexpr (P.cand[k][i]) {
@@ -127,12 +128,11 @@ DENGO_D void block_lu_factor(T (&A)[M][M], T (&rp)[M], int (&piv)[M])
 }
 
 // Solve with a factor from block_lu_factor: x becomes A^{-1} x.
-template <int M, const LUPlan<M> &P, class T>
+template <int M, const LUPlan<M> &P, bool kSkip = false, class T>
 DENGO_D void block_lu_apply(const T (&A)[M][M], const T (&rp)[M], const int (&piv)[M],
                             T (&x)[M])
 {
 #pragma clang fp contract(fast)
-    constexpr bool kSkip = sizeof(T) == sizeof(float);
     static_for<0, M - 1>([&](auto kk) {
         constexpr int k = CV(kk);
         if (__any(piv[k] != k)) {
@@ -298,6 +298,10 @@ template <>
 struct FastTraits<PrimNet> {
     static constexpr int M = prim::M;
     static constexpr const LUPlan<M> &plan = prim::kPlan;
+    // the FP32 factorization exchanges whole candidate rows: skipping
+    // exchanges no lane chose moves this kernel's register allocation into
+    // scratch and costs it about 10%
+    static constexpr bool kSkipExchanges32 = false;
     static DENGO_D bool rhs(const double (&u)[prim::N], double (&f)[prim::N],
                             double (&rs)[prim::NTAB], double &T, double &rmd,
                             const prim::Env &env)
@@ -402,8 +406,8 @@ DENGO_D int sweep_fast(Cell<Net> &c, const KParams<Net> &p)
             });
             y32[i] = (float)(g[i] * d[i]);
         });
-        block_lu_factor<M, FT::plan>(A32, rp, piv);
-        block_lu_apply<M, FT::plan>(A32, rp, piv, y32);
+        block_lu_factor<M, FT::plan, FT::kSkipExchanges32>(A32, rp, piv);
+        block_lu_apply<M, FT::plan, FT::kSkipExchanges32>(A32, rp, piv, y32);
 #pragma unroll
         for (int i = 0; i < M; ++i) y[i] = e[i] * (double)y32[i];
         double tol[M];
@@ -422,7 +426,7 @@ DENGO_D int sweep_fast(Cell<Net> &c, const KParams<Net> &p)
                 });
                 y32[i] = (float)(r * d[i]);
             });
-            block_lu_apply<M, FT::plan>(A32, rp, piv, y32);
+            block_lu_apply<M, FT::plan, FT::kSkipExchanges32>(A32, rp, piv, y32);
             bool small = true;
             double dge = 0.0, yg = g[GE];
 #pragma unroll

@@ -289,6 +289,9 @@ struct FastTraits<MANet<D>> {
     using Net = MANet<D>;
     static constexpr int M = Net::M;
     static constexpr const LUPlan<M> &plan = Net::plan;
+    // the FP32 factorization skips candidate-row exchanges no lane of the
+    // wave chose (one uniform branch per candidate; same pivots)
+    static constexpr bool kSkipExchanges32 = true;
     static DENGO_D bool rhs(const double (&u)[Net::N], double (&f)[Net::N],
                             double (&rs)[Net::NTAB], double &T, double &rmd,
                             const typename Net::Env &env)
