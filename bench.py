@@ -203,11 +203,14 @@ def network_leg(name, solver, x0, rtol, atol, variants, dtf=1e12, T=None, flops_
     for label, (prec, x1, o1) in runs.items():
         ms, x, o = solve(prec, order=runtime.cost_order(ref[1]["nsweep"]))
         same = bool(torch.equal(x, x1)) and all(torch.equal(o[k], o1[k]) for k in o)
-        out[label + "_cost_ordered"] = {
-            "kernel_ms": ms, "cell_solves_per_s": n / (ms * 1e-3),
-            "order": "costly first by the per-cell sweep counts of a previous solve of the same "
-                     "cells (the %s run)" % variants[0][0],
-            "bitwise_equal_to_unordered": same}
+        rec = {"kernel_ms": ms, "cell_solves_per_s": n / (ms * 1e-3),
+               "order": "costly first by the per-cell sweep counts of a previous solve of the same "
+                        "cells (the %s run)" % variants[0][0],
+               "bitwise_equal_to_unordered": same}
+        if flops_per_sweep:
+            rec["fp64_tflops_algorithmic"] = out[variants[0][0]]["sweeps"] * flops_per_sweep / (
+                ms * 1e-3) / 1e12
+        out[label + "_cost_ordered"] = rec
         log("%s %s, cost-ordered: %.1f ms" % (name, label, ms))
     return out
 
@@ -225,6 +228,8 @@ def main():
     ap.add_argument("--also", default="mixed_persistent",
                     help="comma-separated extra variants timed once (empty: none)")
     ap.add_argument("--no-percell", action="store_true", help="skip the plain per-cell kernel")
+    ap.add_argument("--no-cost-ordered", action="store_true",
+                    help="skip the cost-ordered reruns of the oxygen and dense legs")
     ap.add_argument("--cpu-cells", type=int, default=4096)
     ap.add_argument("--cpu-repeats", type=int, default=3,
                     help="CPU baseline runs (the median is reported)")
@@ -392,7 +397,7 @@ def main():
                                     [("fp64_persistent", runtime.FP64),
                                      ("fp64fast_persistent", runtime.FP64_FAST),
                                      ("mixed_persistent", runtime.MIXED)],
-                                    ordered=("mixed_persistent",))}
+                                    ordered=() if args.no_cost_ordered else ("mixed_persistent",))}
         oxy.close()
         del xo, rto, ato
     if args.synthetic_cells > 0:
@@ -406,7 +411,9 @@ def main():
                         "dtf=1e12 s" % args.synthetic_cells,
             "variants": network_leg("synthetic", syn, xs, rts, ats,
                                     [("fp64fast_persistent", runtime.FP64_FAST)], T=Ts,
-                                    flops_per_sweep=F_SWEEP_DENSE)}
+                                    flops_per_sweep=F_SWEEP_DENSE,
+                                    ordered=() if args.no_cost_ordered
+                                    else ("fp64fast_persistent",))}
         syn.close()
         del xs, Ts, rts, ats
 
