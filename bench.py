@@ -153,14 +153,17 @@ def parity_stats(x, st, nsub, nrej, x_ref, st_ref, nsub_ref, nrej_ref):
 
 
 def network_leg(name, solver, x0, rtol, atol, variants, dtf=1e12, T=None, flops_per_sweep=None,
-                ordered=()):
+                ordered=(), density_ordered=(), nspecies=None):
     """Runs every variant once on device copies of the same inputs; the first
     variant is the parity reference.  Returns per-variant kernel time,
     cell-solves/s and parity statistics.  Each variant named in `ordered` runs
     once more with the cells handed out costly-first by the reference run's
     per-cell sweep counts (runtime.cost_order: the order a caller builds from
     a previous solve of the same cells); its results are checked bit for bit
-    against the variant's own unordered run."""
+    against the variant's own unordered run.  Each variant named in
+    `density_ordered` reruns with the densest cells first
+    (runtime.density_order over the first `nspecies` columns), an order that
+    needs no previous solve."""
     import torch
     from dengo_amd import runtime
     dev = torch.device("cuda:0")
@@ -187,7 +190,7 @@ def network_leg(name, solver, x0, rtol, atol, variants, dtf=1e12, T=None, flops_
                "status_counts": torch.bincount(o["status"].long(), minlength=3).tolist(),
                "cell_sweeps_p99_9": float(torch.quantile(sw[:1 << 24], 0.999)) if n else 0.0,
                "cell_sweeps_max": int(o["nsweep"].max()) if n else 0}
-        if label in ordered:
+        if label in ordered or label in density_ordered:
             runs[label] = (prec, x, o)
         if ref is None:
             ref = (x, o)
@@ -200,18 +203,25 @@ def network_leg(name, solver, x0, rtol, atol, variants, dtf=1e12, T=None, flops_
                                     ref[1]["status"], ref[1]["nsub"], ref[1]["nrej"]))
         out[label] = rec
         log("%s %s: %.1f ms" % (name, label, ms))
-    for label, (prec, x1, o1) in runs.items():
-        ms, x, o = solve(prec, order=runtime.cost_order(ref[1]["nsweep"]))
+    reruns = [(label, "cost_ordered", "costly first by the per-cell sweep counts of a previous "
+               "solve of the same cells (the %s run)" % variants[0][0]) for label in ordered]
+    reruns += [(label, "density_ordered", "densest cells first (total density of the inputs; "
+                "no previous solve)") for label in density_ordered]
+    for label, tag, how in reruns:
+        prec, x1, o1 = runs[label]
+        if tag == "cost_ordered":
+            order = runtime.cost_order(ref[1]["nsweep"])
+        else:
+            order = runtime.density_order(torch.from_numpy(x0).to(dev), nspecies)
+        ms, x, o = solve(prec, order=order)
         same = bool(torch.equal(x, x1)) and all(torch.equal(o[k], o1[k]) for k in o)
-        rec = {"kernel_ms": ms, "cell_solves_per_s": n / (ms * 1e-3),
-               "order": "costly first by the per-cell sweep counts of a previous solve of the same "
-                        "cells (the %s run)" % variants[0][0],
+        rec = {"kernel_ms": ms, "cell_solves_per_s": n / (ms * 1e-3), "order": how,
                "bitwise_equal_to_unordered": same}
         if flops_per_sweep:
             rec["fp64_tflops_algorithmic"] = out[variants[0][0]]["sweeps"] * flops_per_sweep / (
                 ms * 1e-3) / 1e12
-        out[label + "_cost_ordered"] = rec
-        log("%s %s, cost-ordered: %.1f ms" % (name, label, ms))
+        out["%s_%s" % (label, tag)] = rec
+        log("%s %s, %s: %.1f ms" % (name, label, tag.replace("_", "-"), ms))
     return out
 
 
@@ -229,7 +239,7 @@ def main():
                     help="comma-separated extra variants timed once (empty: none)")
     ap.add_argument("--no-percell", action="store_true", help="skip the plain per-cell kernel")
     ap.add_argument("--no-cost-ordered", action="store_true",
-                    help="skip the cost-ordered reruns of the oxygen and dense legs")
+                    help="skip the ordered reruns (cost and density) of the oxygen and dense legs")
     ap.add_argument("--cpu-cells", type=int, default=4096)
     ap.add_argument("--cpu-repeats", type=int, default=3,
                     help="CPU baseline runs (the median is reported)")
@@ -397,7 +407,9 @@ def main():
                                     [("fp64_persistent", runtime.FP64),
                                      ("fp64fast_persistent", runtime.FP64_FAST),
                                      ("mixed_persistent", runtime.MIXED)],
-                                    ordered=() if args.no_cost_ordered else ("mixed_persistent",))}
+                                    ordered=() if args.no_cost_ordered else ("mixed_persistent",),
+                                    density_ordered=() if args.no_cost_ordered
+                                    else ("mixed_persistent",), nspecies=10)}
         oxy.close()
         del xo, rto, ato
     if args.synthetic_cells > 0:
@@ -413,6 +425,8 @@ def main():
                                     [("fp64fast_persistent", runtime.FP64_FAST)], T=Ts,
                                     flops_per_sweep=F_SWEEP_DENSE,
                                     ordered=() if args.no_cost_ordered
+                                    else ("fp64fast_persistent",),
+                                    density_ordered=() if args.no_cost_ordered
                                     else ("fp64fast_persistent",))}
         syn.close()
         del xs, Ts, rts, ats

@@ -259,6 +259,19 @@ def cost_order(nsweep):
     return torch.argsort(nsweep.to(torch.int64), descending=True, stable=True).contiguous()
 
 
+def density_order(x, nspecies=None):
+    """The hand-out order that puts the densest cells first: cell indices by
+    descending total density, the sum of the first `nspecies` columns of x
+    (default: every column; pass N - 1 for networks whose last unknown is
+    the energy).  It needs no previous solve.  A cell's Newton work grows
+    with its density on the mass-action networks, so handing out the costly
+    cells early shortens the persistent schedule's tail.  Returns an int64
+    tensor on x's device."""
+    import torch
+    xs = x if nspecies is None else x[:, :nspecies]
+    return torch.argsort(xs.sum(dim=1), descending=True, stable=True).contiguous()
+
+
 def write_table_blob(path, block, T_bounds, z_bounds=(0.0, 0.0)):
     """Write the DENGOTB1 table file read by <name>_setup_data
     (dengo_amd.tables.write_blob)."""

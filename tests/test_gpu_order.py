@@ -2,7 +2,8 @@
 depend on it, bit for bit, on every network and both schedules, eager and
 captured; an order that is not a permutation of the cells is refused before
 anything runs; runtime.cost_order builds the costly-first order from a
-previous call's sweep counts."""
+previous call's sweep counts, runtime.density_order the densest-first order
+from the inputs."""
 import os
 
 import numpy as np
@@ -107,6 +108,8 @@ def test_oxygen_order(oxy):
     ref = oxy.evolve(inp, rtol, atol, 1e11, precision=runtime.MIXED)
     perm = torch.from_numpy(np.random.default_rng(5).permutation(256)).to(DEV)
     _same(oxy.evolve(inp, rtol, atol, 1e11, precision=runtime.MIXED, order=perm), ref)
+    dense_first = runtime.density_order(torch.from_numpy(inp).to(DEV), 10)
+    _same(oxy.evolve(inp, rtol, atol, 1e11, precision=runtime.MIXED, order=dense_first), ref)
 
 
 def test_dense_order(dense):
