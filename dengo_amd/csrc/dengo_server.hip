@@ -12,7 +12,9 @@
 //   idle    with nothing in flight and nothing waiting the worker sleeps on a
 //           condition variable; otherwise it polls its events every ~20 us.
 // Clients only queue requests (submit) and collect completion records (poll).
+#include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <deque>
 #include <mutex>
@@ -55,6 +57,9 @@ struct Slot {
     long long off = -1;
     double *dx = nullptr, *drt = nullptr, *dat = nullptr, *dT = nullptr;
     int *dst = nullptr;
+    long long *dord = nullptr;         // dense network: the hand-out order
+    std::vector<long long> hord;       // its host copy, alive until the slot completes
+    std::vector<double> hkey;
     double t_admit = 0;
     int inflight_at_admit = 0, max_inflight = 0;
 };
@@ -103,7 +108,8 @@ struct dengo_server {
     {
         const size_t row = (size_t)n * hooks[net].nspecies * sizeof(double);
         size_t b = 3 * round_up(row) + round_up((size_t)n * sizeof(int));
-        if (hooks[net].isothermal) b += round_up((size_t)n * sizeof(double));
+        // the dense network's patches also carry T[n] and the order[n]
+        if (hooks[net].isothermal) b += 2 * round_up((size_t)n * sizeof(double));
         return b;
     }
 
@@ -162,6 +168,24 @@ struct dengo_server {
         if (s.dT)
             DENGO_HIP_OK(hipMemcpyAsync(s.dT, q.T, (size_t)q.n * sizeof(double),
                                         hipMemcpyHostToDevice, s.stream));
+        if (s.dord) {
+            // densest cells first (total density of the species columns; a
+            // NaN total sorts last), ties in index order
+            const int ns = hooks[q.net].nspecies;
+            s.hord.resize((size_t)q.n);
+            s.hkey.resize((size_t)q.n);
+            for (long long i = 0; i < q.n; ++i) {
+                double t = 0.0;
+                for (int j = 0; j < ns; ++j) t += q.x[(size_t)i * ns + j];
+                s.hkey[(size_t)i] = std::isnan(t) ? -HUGE_VAL : t;
+                s.hord[(size_t)i] = i;
+            }
+            std::stable_sort(s.hord.begin(), s.hord.end(), [&s](long long a, long long b) {
+                return s.hkey[(size_t)a] > s.hkey[(size_t)b];
+            });
+            DENGO_HIP_OK(hipMemcpyAsync(s.dord, s.hord.data(), (size_t)q.n * sizeof(long long),
+                                        hipMemcpyHostToDevice, s.stream));
+        }
         DENGO_HIP_OK(hipEventRecord(s.ev_start, s.stream));
         LaunchArgs a;
         memset(&a, 0, sizeof(a));
@@ -181,6 +205,7 @@ struct dengo_server {
         a.out.status = s.dst;
         a.stream = s.stream;
         a.counter = s.counter;
+        a.opt.order = s.dord;
         if (hooks[q.net].launch(data[q.net], a)) return -1;
         DENGO_HIP_OK(hipEventRecord(s.ev_end, s.stream));
         return 0;
@@ -265,9 +290,11 @@ struct dengo_server {
                 s.drt = (double *)(p + row);
                 s.dat = (double *)(p + 2 * row);
                 s.dst = (int *)(p + 3 * row);
-                s.dT = hooks[s.req.net].isothermal
-                           ? (double *)(p + 3 * row + round_up((size_t)s.req.n * sizeof(int)))
-                           : nullptr;
+                const size_t st_end = 3 * row + round_up((size_t)s.req.n * sizeof(int));
+                s.dT = hooks[s.req.net].isothermal ? (double *)(p + st_end) : nullptr;
+                s.dord = hooks[s.req.net].isothermal && cfg.dense_order
+                             ? (long long *)(p + st_end + round_up((size_t)s.req.n * sizeof(double)))
+                             : nullptr;
                 s.phase = kRunning;
                 s.t_admit = now();
                 inflight++;
@@ -303,6 +330,7 @@ void dengo_server_default_config(dengo_server_config *cfg)
     cfg->max_bypass = 8;
     for (int k = 0; k < 3; ++k) cfg->precision[k] = -1;
     for (int k = 0; k < 3; ++k) cfg->grid_shares[k] = 0;
+    cfg->dense_order = 1;
 }
 
 dengo_server *dengo_server_create(const dengo_server_config *cfg_in, void *primordial,

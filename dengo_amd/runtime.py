@@ -42,7 +42,7 @@ class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_size_t), ("slots", ctypes.c_int),
                 ("grid_share", ctypes.c_int), ("max_bypass", ctypes.c_int),
                 ("precision", ctypes.c_int * 3), ("grid_shares", ctypes.c_int * 3),
-                ("reserved", ctypes.c_int)]
+                ("dense_order", ctypes.c_int)]
 
 
 class PatchResult(ctypes.Structure):
@@ -574,9 +574,11 @@ class PatchServer(object):
 
     def __init__(self, primordial=None, oxygen=None, synthetic=None, precision=MIXED,
                  hbm_budget=8 << 30, slots=4, grid_share=0, max_bypass=8, net_precision=None,
-                 net_grid_share=None):
+                 net_grid_share=None, dense_order=True):
         """net_precision: optional {network: precision} overriding precision;
-        net_grid_share: optional {network: grid share} overriding grid_share."""
+        net_grid_share: optional {network: grid share} overriding grid_share;
+        dense_order: hand out each dense-network patch's cells densest first
+        (results do not depend on it)."""
         lib = load()
         cfg = ServerConfig()
         lib.dengo_server_default_config(ctypes.byref(cfg))
@@ -588,6 +590,7 @@ class PatchServer(object):
             cfg.precision[k] = v
         for k, v in (net_grid_share or {}).items():
             cfg.grid_shares[k] = v
+        cfg.dense_order = 1 if dense_order else 0
         self._solvers = (primordial, oxygen, synthetic)
         ptr = [None if x is None else x.data for x in self._solvers]
         self.srv = lib.dengo_server_create(ctypes.byref(cfg), ptr[0], ptr[1], ptr[2], precision)

@@ -56,7 +56,10 @@ typedef struct dengo_server_config {
     int grid_shares[3];   /* per network: grid_share for its patches, or 0 for
                              grid_share (a network whose patches are long may
                              take a larger share of the GPU) */
-    int reserved;
+    int dense_order;      /* 1 (default): the dense network's kernel hands out
+                             each patch's cells densest first (a cell's Newton
+                             work grows with its density); 0: in index order.
+                             Results do not depend on it. */
 } dengo_server_config;
 
 /* Completion record of one request. */

@@ -29,9 +29,10 @@ def round_up(n):
 
 
 def request_bytes(net, n):
-    """Arena bytes of one patch: state, rtol, atol, status (and T)."""
+    """Arena bytes of one patch: state, rtol, atol, status (and T and the
+    hand-out order)."""
     b = 3 * round_up(n * NSPEC[net] * 8) + round_up(n * 4)
-    return b + (round_up(n * 8) if net == SYN else 0)
+    return b + (2 * round_up(n * 8) if net == SYN else 0)
 
 
 @pytest.fixture(scope="module")
@@ -78,7 +79,8 @@ def alone(nets, net, p, precision):
 
 def serve(nets, plist, precision, **cfg):
     """Submits every patch at once; returns (per-patch (state, atol, status),
-    completion records by ticket, stats)."""
+    completion records by ticket, stats).  The dense network's patches run
+    densest first unless dense_order=False is given."""
     prim, oxy, syn = nets
     srv = runtime.PatchServer(prim, oxy, syn, precision=precision, **cfg)
     try:
@@ -148,8 +150,10 @@ def test_per_network_precision(nets):
 
 
 def test_one_slot_server_equals_mix(nets):
-    plist = patches()[:4]
-    one, recs1, st1 = serve(nets, plist, runtime.MIXED, slots=1)
+    """One slot against four, the dense patch handed out in index order
+    against densest first: the same bits."""
+    plist = patches()[:4] + patches()[-1:]
+    one, recs1, st1 = serve(nets, plist, runtime.MIXED, slots=1, dense_order=False)
     four, _, _ = serve(nets, plist, runtime.MIXED, slots=4)
     for a, b in zip(one, four):
         for x, y in zip(a, b):

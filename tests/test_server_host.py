@@ -31,6 +31,7 @@ def test_server_config_defaults():
     assert cfg.hbm_budget == 8 << 30 and cfg.slots == 4 and cfg.max_bypass == 8
     assert cfg.grid_share == 0 and list(cfg.precision) == [-1, -1, -1]
     assert list(cfg.grid_shares) == [0, 0, 0]
+    assert cfg.dense_order == 1
 
 
 def test_server_create_rejects_bad_arguments():
