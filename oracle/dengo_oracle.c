@@ -340,6 +340,21 @@ void orc_prim_interpolate(orc_prim_data *d)
     }
 }
 
+/* orc_prim_interpolate at given ln T values, for ntab tables of nT bins on
+ * [T0, T1]: out[i * ntab + t] (test access to the interpolation alone). */
+void orc_interp_logT(const double *tables, int ntab, int nT, double T0, double T1,
+                     const double *logT, long long n, double *out)
+{
+    for (long long i = 0; i < n; ++i) {
+        orc_prim_data d;
+        orc_prim_setup(&d, tables, nT, T0, T1);
+        d.ntab = ntab;
+        d.logT = logT[i];
+        orc_prim_interpolate(&d);
+        memcpy(out + i * ntab, d.rs, sizeof(double) * (size_t)ntab);
+    }
+}
+
 /* Cooling sum in action order brem, ceHeI, ceHeII, ceHI, ciHeI, ciHeII,
  * ciHeIS, ciHI, compton, gammah (identically 0), gloverabel08, h2formation,
  * reHeII1, reHeII2, reHeIII, reHII (dengo/primordial_cooling.py:34-316). */

@@ -94,6 +94,11 @@ int orc_prim_jac(double *input, double *J, int nstrip, int nchem, void *sdata);
 void orc_prim_temperature(orc_prim_data *d, const double *u);
 void orc_prim_interpolate(orc_prim_data *d);
 
+/* orc_prim_interpolate at given ln T values: out[i * ntab + t] for ntab
+ * tables of nT bins on [T0, T1] (ntab <= ORC_MAXTAB). */
+void orc_interp_logT(const double *tables, int ntab, int nT, double T0, double T1,
+                     const double *logT, long long n, double *out);
+
 /* Whole-step driver over `dims` independent cells.  `input`, `rtol`, `atol`
  * are [dims][10] as in dengo_evolve; input and atol are updated in place.
  * dt: in = initial substep (<0 selects dtf/1e5), out = final dt of the cell

@@ -79,6 +79,23 @@ def gauss_elim(A, b):
 PORTABLE_LOG = 1
 
 
+def interp_logT(tables, logT, T_bounds=(1.0, 1e9)):
+    """The oracle's table interpolation alone at the given ln T values:
+    rates[len(logT), ntab]."""
+    lib = load()
+    if not getattr(lib, "_interp_ready", False):
+        lib.orc_interp_logT.restype = None
+        lib.orc_interp_logT.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                        ctypes.c_double, _dp, ctypes.c_longlong, _dp]
+        lib._interp_ready = True
+    tab = np.ascontiguousarray(tables, dtype=np.float64)
+    lt = np.ascontiguousarray(logT, dtype=np.float64)
+    out = np.zeros((lt.size, tab.shape[0]))
+    lib.orc_interp_logT(_ptr(tab), tab.shape[0], tab.shape[1], T_bounds[0], T_bounds[1],
+                        _ptr(lt), lt.size, _ptr(out))
+    return out
+
+
 def prim_eval(u, tables, T_bounds=(1.0, 1e9), z=-1.0, flags=0):
     """RHS f[n,10], Jacobian J[n,10,10] (J[c, col, row]), T[n], rates[n,45]."""
     u = np.ascontiguousarray(u, dtype=np.float64)

@@ -24,7 +24,7 @@ def test_ion_rhs_conservation_and_temperature():
     charge = (f[:, :9] * np.arange(9)).sum(axis=1) - f[:, 9]
     assert np.all(np.abs(charge) <= 1e-12 * mag)
     inside = (T > 1e4) & (T < 1e9)
-    assert np.allclose(T2[inside], T[inside], rtol=1e-12)
+    assert np.allclose(T2[inside], T[inside], rtol=1e-12, atol=0)
     # electrons recombine onto ion k at rate k_r: the ion chain's rate law
     k2r = rates[:, 2]        # table 2 is O_2_r (O_2 + e -> O_1)
     k1i = rates[:, 0]        # table 0 is O_1_i (O_1 + e -> O_2 + 2e)
