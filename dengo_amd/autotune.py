@@ -17,9 +17,12 @@ workload:
 Timings are kernel times from HIP events on the launch stream.  The sample
 should be as large as the batches the choice is for: the variants' ranking
 depends on load, and a small sample is timed by its longest cell.  Results can be
-cached in a JSON file keyed by device, network, unknowns and sample size, so a
-simulation tunes once and reuses the choice every step.
+cached in a JSON file keyed by device, network, unknowns, sample size, dtf and
+a digest of the network's tables and the sample's tolerances (whether a fast
+variant meets the bar depends on all of them), so a simulation tunes once and
+reuses the choice every step for the same dtf and data.
 """
+import hashlib
 import json
 import os
 
@@ -34,6 +37,22 @@ def _name(solver):
     if isinstance(solver, runtime.SyntheticSolver):
         return "synthetic%d" % solver.N
     return getattr(solver, "name", type(solver).__name__)
+
+
+def _digest(solver, rt, at, T):
+    """Digest of what the accuracy gate depends on besides dtf: the network's
+    tables (or reaction data) and the sample's tolerances and temperatures."""
+    h = hashlib.sha1()
+    if isinstance(solver, runtime.SyntheticSolver):
+        for a in (solver.net.rxn, solver.net.A, solver.net.beta, solver.net.gamma):
+            h.update(np.ascontiguousarray(a).tobytes())
+    else:
+        h.update(np.ascontiguousarray(solver.tables).tobytes())
+        h.update(np.asarray(solver.T_bounds, dtype=np.float64).tobytes())
+    for t in (rt, at, T):
+        if t is not None:
+            h.update(t.detach().cpu().numpy().tobytes())
+    return h.hexdigest()[:16]
 
 
 def _precisions(solver):
@@ -77,14 +96,15 @@ def tune(solver, x, rtol, atol, dtf, T=None, sample=4096, rtol_bar=1e-6, min_agr
     import torch
     n = min(sample, x.shape[0])
     dev = torch.cuda.get_device_properties(x.device)
-    key = "%s|%s|%d|%d|%g|%g" % (dev.name, _name(solver), solver.N, n, rtol_bar, min_agree)
+    xs, rt, at = x[:n].contiguous(), rtol[:n].contiguous(), atol[:n].contiguous()
+    Ts = None if T is None else T[:n].contiguous()
+    key = "%s|%s|%d|%d|%g|%g|dtf=%r|%s" % (dev.name, _name(solver), solver.N, n, rtol_bar,
+                                           min_agree, float(dtf), _digest(solver, rt, at, Ts))
     if cache and os.path.exists(cache):
         with open(cache) as f:
             saved = json.load(f)
         if key in saved:
             return saved[key]
-    xs, rt, at = x[:n].contiguous(), rtol[:n].contiguous(), atol[:n].contiguous()
-    Ts = None if T is None else T[:n].contiguous()
 
     def timed(**opt):
         ms = []

@@ -12,6 +12,10 @@ struct dengo_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     unsigned long long *counter = nullptr;
+    // gather/scatter rows and outputs of a cell order: owned by the graph, so
+    // later calls on the solver (which resize the solver's own buffers) and
+    // other graphs never touch the addresses this graph's kernels use
+    OrderBufs ord;
 };
 
 extern "C" {
@@ -22,6 +26,7 @@ void dengo_graph_destroy(dengo_graph *g)
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
     if (g->counter) (void)hipFree(g->counter);
+    order_free(g->ord);
     delete g;
 }
 
@@ -52,8 +57,6 @@ dengo_graph *dengo_graph_create(int network, void *data, const dengo_options *op
     o.block = 0;
     int full = 0;
     if (h.prepare(data, &o, &full)) return nullptr;   // tables resident before capture
-    // a cell order is checked once, here; replays use the same array
-    if (o.order && h.prepare_order(data, o.order, dims, nullptr)) return nullptr;
     dengo_graph *g = new dengo_graph();
     hipStream_t st = nullptr;
     auto fail = [&](const char *what, hipError_t e) -> dengo_graph * {
@@ -62,6 +65,9 @@ dengo_graph *dengo_graph_create(int network, void *data, const dengo_options *op
         dengo_graph_destroy(g);
         return nullptr;
     };
+    // a cell order is checked once, here; replays use the same array
+    if (o.order && h.prepare_order(data, &g->ord, o.order, dims, nullptr))
+        return fail("", hipSuccess);
     hipError_t e;
     if ((e = hipMalloc(&g->counter, 256)) != hipSuccess) return fail("hipMalloc", e);
     if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess)
@@ -81,6 +87,7 @@ dengo_graph *dengo_graph_create(int network, void *data, const dengo_options *op
     if (outputs) a.out = *outputs;
     a.stream = st;
     a.counter = g->counter;
+    a.ord = &g->ord;
     if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess)
         return fail("hipStreamBeginCapture", e);
     const int rc = h.launch(data, a);

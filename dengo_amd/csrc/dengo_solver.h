@@ -229,8 +229,11 @@ const char *last_error_text();
                                       __FILE__, __LINE__);                               \
     } while (0)
 
-// One integration enqueued by the patch server: a network-independent view of
-// the arguments of <n>_evolve_device with the queue counter made explicit.
+struct OrderBufs;
+
+// One integration enqueued by the patch server or captured in a graph: a
+// network-independent view of the arguments of <n>_evolve_device with the
+// queue counter made explicit.
 struct LaunchArgs {
     dengo_options opt;
     double dtf, dt0, z;
@@ -242,6 +245,7 @@ struct LaunchArgs {
     dengo_cell_outputs out;
     hipStream_t stream;
     unsigned long long *counter;  // persistent schedule's work counter
+    OrderBufs *ord;               // buffers of opt.order (nullptr: the solver's own)
 };
 
 // What the patch server needs of a network: prepare() makes the tables
@@ -251,8 +255,10 @@ struct NetHooks {
     int nspecies;
     int isothermal;               // 1: cells carry T[n] instead of an energy unknown
     int (*prepare)(void *data, const dengo_options *opt, int *full_grid);
-    // buffers for and check of a cell order of n cells (synchronises stream)
-    int (*prepare_order)(void *data, const long long *order, long long n, void *stream);
+    // sizes the buffers b (nullptr: the solver's own) for a cell order of n
+    // cells and checks the order (synchronises stream)
+    int (*prepare_order)(void *data, OrderBufs *b, const long long *order, long long n,
+                         void *stream);
     int (*launch)(void *data, const LaunchArgs &a);
 };
 NetHooks hooks_primordial();
@@ -856,24 +862,26 @@ struct Host {
     static int launch(Data *d, DevState *s, const dengo_options &o, double dtf, double dt0,
                       double z, double *d_input, const double *d_rtol, double *d_atol,
                       long long dims, const dengo_cell_outputs &out, hipStream_t stream,
-                      unsigned long long *counter, bool events)
+                      unsigned long long *counter, bool events, OrderBufs *ob = nullptr)
     {
         if (!o.order || dims <= 0)
             return launch_cells(d, s, o, dtf, dt0, z, d_input, d_rtol, d_atol, dims, out, stream,
                                 counter, events);
         // cells in hand-out order: gather, integrate, scatter back (the
         // buffers were allocated and the order checked by the caller)
-        OrderBufs &b = s->ord;
+        OrderBufs &b = ob ? *ob : s->ord;
         if (b.cap < dims || b.n < N) return set_error("cell order buffers not prepared");
         dengo_options oc = o;
         oc.order = nullptr;
         const dengo_cell_outputs pout = order_outputs(b, out);
         const unsigned gr = (unsigned)((dims * N + 255) / 256), gc = (unsigned)((dims + 255) / 256);
+        // the timed interval covers the gather and the scatters too
+        if (events) DENGO_HIP_OK(hipEventRecord(s->ev0, stream));
         hipLaunchKernelGGL((k_order_rows<N, true>), dim3(gr), dim3(256), 0, stream, o.order, dims,
                            d_input, d_rtol, d_atol, b.in, b.rt, b.at);
         DENGO_HIP_OK(hipGetLastError());
         if (launch_cells(d, s, oc, dtf, dt0, z, b.in, b.rt, b.at, dims, pout, stream, counter,
-                         events))
+                         false))
             return -1;
         hipLaunchKernelGGL((k_order_rows<N, false>), dim3(gr), dim3(256), 0, stream, o.order, dims,
                            b.in, b.at, nullptr, d_input, d_atol, nullptr);
@@ -881,6 +889,7 @@ struct Host {
         hipLaunchKernelGGL(k_order_outputs<0>, dim3(gc), dim3(256), 0, stream, o.order, dims, pout,
                            out);
         DENGO_HIP_OK(hipGetLastError());
+        if (events) DENGO_HIP_OK(hipEventRecord(s->ev1, stream));
         return 0;
     }
 
@@ -1192,15 +1201,15 @@ struct Host {
         return full_grid(s, oo, full);
     }
 
-    static int hook_prepare_order(void *data, const long long *order, long long n, void *stream)
+    static int hook_prepare_order(void *data, OrderBufs *ob, const long long *order, long long n,
+                                  void *stream)
     {
         Data *d = (Data *)data;
         DevState *s = state(d);
         if (ensure_device(d, s)) return -1;
         if (!order || n <= 0) return 0;
-        return order_alloc(s->ord, n, N) || order_check(s->ord, order, n, (hipStream_t)stream)
-                   ? -1
-                   : 0;
+        OrderBufs &b = ob ? *ob : s->ord;
+        return order_alloc(b, n, N) || order_check(b, order, n, (hipStream_t)stream) ? -1 : 0;
     }
 
     static int hook_launch(void *data, const LaunchArgs &a)
@@ -1209,7 +1218,7 @@ struct Host {
         dengo_options o;
         if (check_options(&a.opt, &o)) return -1;
         return launch(d, state(d), o, a.dtf, a.dt0, a.z, a.x, a.rtol, a.atol, a.n, a.out,
-                      a.stream, a.counter, false);
+                      a.stream, a.counter, false, a.ord);
     }
 
     static NetHooks hooks()

@@ -202,12 +202,14 @@ int hook_prepare(void *data, const dengo_options *opt, int *full)
     return 0;
 }
 
-int hook_prepare_order(void *data, const long long *order, long long n, void *stream)
+int hook_prepare_order(void *data, OrderBufs *ob, const long long *order, long long n,
+                       void *stream)
 {
     synthetic_data *d = (synthetic_data *)data;
     if (ensure_device(d)) return -1;
     if (!order || n <= 0) return 0;
-    return order_alloc(d->ord, n, 0) || order_check(d->ord, order, n, (hipStream_t)stream) ? -1 : 0;
+    OrderBufs &bufs = ob ? *ob : d->ord;   // the dense kernel reads the order itself
+    return order_alloc(bufs, n, 0) || order_check(bufs, order, n, (hipStream_t)stream) ? -1 : 0;
 }
 
 int hook_launch(void *data, const LaunchArgs &a)

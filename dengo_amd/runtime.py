@@ -564,6 +564,37 @@ class PinnedArray(object):
             pass
 
 
+def check_patch(nspecies, isothermal, state, rtol, atol, T=None, status=None):
+    """Validates one patch of host arrays before the server copies n * N
+    values in and out of them: float64 state/rtol/atol of shape (n, N), T
+    float64 (n,) for an isothermal network (and absent otherwise), status
+    int32 (n,), all C-contiguous numpy arrays.  Returns the arrays to keep
+    alive until completion."""
+    def need(a, name, dtype, shape):
+        if not isinstance(a, np.ndarray):
+            raise ValueError("%s must be a numpy array" % name)
+        if a.dtype != np.dtype(dtype):
+            raise ValueError("%s must be %s, got %s" % (name, np.dtype(dtype).name, a.dtype))
+        if a.shape != shape:
+            raise ValueError("%s must have shape %s, got %s" % (name, shape, a.shape))
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("%s must be C-contiguous" % name)
+    if not isinstance(state, np.ndarray) or state.ndim != 2:
+        raise ValueError("state must be a 2-d numpy array [n, %d]" % nspecies)
+    n = state.shape[0]
+    for a, name in ((state, "state"), (rtol, "rtol"), (atol, "atol")):
+        need(a, name, np.float64, (n, nspecies))
+    if isothermal:
+        if T is None:
+            raise ValueError("this network needs per-cell temperatures T[n]")
+        need(T, "T", np.float64, (n,))
+    elif T is not None:
+        raise ValueError("T is only taken by the isothermal network")
+    if status is not None:
+        need(status, "status", np.int32, (n,))
+    return [state, rtol, atol] + [a for a in (T, status) if a is not None]
+
+
 class PatchServer(object):
     """Continuous batching of patch solves on one GPU (include/dengo_server.h).
 
@@ -602,10 +633,12 @@ class PatchServer(object):
         """Queue one patch; state and atol [n, N] are updated in place at
         completion, status[n] (int32) receives per-cell status.  Returns the
         ticket."""
-        arrays = [state, rtol, atol] + [a for a in (T, status) if a is not None]
-        for a in arrays:
-            if not a.flags["C_CONTIGUOUS"]:
-                raise ValueError("patch arrays must be C-contiguous")
+        if network not in (NET_PRIMORDIAL, NET_OXYGEN, NET_SYNTHETIC) or \
+                self._solvers[network] is None:
+            raise ValueError("network %r is not served" % (network,))
+        solver = self._solvers[network]
+        arrays = check_patch(solver.N, isinstance(solver, SyntheticSolver), state, rtol, atol, T,
+                             status)
         n = state.shape[0]
         t = load().dengo_server_submit(
             self.srv, network, n, state.ctypes.data, rtol.ctypes.data, atol.ctypes.data,

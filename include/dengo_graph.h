@@ -8,9 +8,12 @@
  * sequence -- the queue-counter reset and the persistent kernel -- once at
  * capture instead of every step.  Each replay integrates whatever the
  * buffers hold at that moment, in place, with the options, dtf, dt0 and z of
- * the capture.  The graph owns its queue counter: replays of one graph must
- * not overlap (replays of different graphs may).  The captured sequence
- * contains kernels only (no memset or copy nodes).
+ * the capture.  The graph owns its queue counter and, when opt->order is
+ * given, its own gather/scatter buffers for that order (sized at creation),
+ * so later calls on the same data struct and other graphs do not disturb
+ * it: replays of one graph must not overlap (replays of different graphs
+ * may).  The captured sequence contains kernels only (no memset or copy
+ * nodes).
  */
 #ifndef DENGO_GRAPH_H
 #define DENGO_GRAPH_H

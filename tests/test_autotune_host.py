@@ -16,3 +16,22 @@ def test_agreement_counts_status_and_relative_error():
     assert abs(wb - 2.0 / 3.0) < 1e-12   # over the three cells both runs finish
     # nothing finished in both runs: the bar holds vacuously
     assert agreement(ref, st_ref + 1, x, st_ref + 1, 1e-6) == (1.0, 1.0)
+
+
+def test_cache_digest_covers_tables_and_tolerances():
+    from types import SimpleNamespace
+
+    import numpy as np
+
+    from dengo_amd.autotune import _digest
+    tab = np.ones((3, 8))
+    s1 = SimpleNamespace(tables=tab, T_bounds=(1.0, 1e9))
+    s2 = SimpleNamespace(tables=tab * 2, T_bounds=(1.0, 1e9))
+    rt = torch.full((4, 3), 1e-9, dtype=torch.float64)
+    at = torch.full((4, 3), 1e-20, dtype=torch.float64)
+    d = _digest(s1, rt, at, None)
+    assert d == _digest(s1, rt.clone(), at.clone(), None)
+    assert d != _digest(s2, rt, at, None)                 # other tables
+    assert d != _digest(s1, rt * 10, at, None)            # other rtol
+    assert d != _digest(s1, rt, at * 10, None)            # other atol scale
+    assert d != _digest(SimpleNamespace(tables=tab, T_bounds=(1e4, 1e9)), rt, at, None)

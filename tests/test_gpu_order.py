@@ -102,6 +102,44 @@ def test_captured_launch_with_order(solver):
     g.close()
 
 
+def test_two_ordered_graphs_survive_larger_eager_calls(solver):
+    # each graph gathers and scatters through buffers of its own: an eager
+    # ordered call on more cells (which regrows the solver's buffers) and a
+    # second ordered graph between replays leave every replay bitwise equal
+    # to the eager result
+    sizes = (96, 160)
+    graphs, refs, bufs = [], [], []
+    for k, n in enumerate(sizes):
+        inp, _, _ = primordial_ics(n, seed=60 + k)
+        rtol, atol = tolerances(inp)
+        ref = solver.evolve(inp, rtol, atol, 1e9, precision=runtime.FP64_FAST, max_substeps=3000)
+        order = torch.from_numpy(np.random.default_rng(k).permutation(n)).to(DEV)
+        x = torch.from_numpy(inp).to(DEV)
+        rt = torch.from_numpy(rtol).to(DEV)
+        at = torch.from_numpy(atol).to(DEV)
+        graphs.append(runtime.CapturedEvolve(solver, x, rt, at, 1e9, precision=runtime.FP64_FAST,
+                                             max_substeps=3000, order=order))
+        refs.append(ref)
+        bufs.append((inp, atol, x, at))
+    big, _, _ = primordial_ics(700, seed=69)
+    brt, bat = tolerances(big)
+    bref = solver.evolve(big, brt, bat, 1e9, precision=runtime.FP64_FAST, max_substeps=3000)
+    border = torch.from_numpy(np.random.default_rng(9).permutation(700)).to(DEV)
+    for rep in range(3):
+        for g, ref, (inp, atol, x, at) in zip(graphs, refs, bufs):
+            x.copy_(torch.from_numpy(inp))
+            at.copy_(torch.from_numpy(atol))
+            g.launch()
+            torch.cuda.synchronize()
+            assert np.array_equal(x.cpu().numpy(), ref["state"]), rep
+            assert np.array_equal(at.cpu().numpy(), ref["atol"]), rep
+        # an eager ordered call on more cells between replays
+        _same(solver.evolve(big, brt, bat, 1e9, precision=runtime.FP64_FAST, max_substeps=3000,
+                            order=border), bref)
+    for g in graphs:
+        g.close()
+
+
 def test_oxygen_order(oxy):
     inp, _, _ = oxygen_ics(256, seed=43)
     rtol, atol = tolerances(inp)

@@ -51,3 +51,33 @@ def test_server_create_rejects_bad_arguments():
     cfg.grid_shares[2] = -1
     assert not lib.dengo_server_create(cfg, None, None, None, runtime.MIXED)
     assert "grid_shares[2]" in runtime.last_error()
+
+
+def test_check_patch_refuses_arrays_the_server_would_overrun():
+    import numpy as np
+    ok = dict(state=np.zeros((5, 10)), rtol=np.zeros((5, 10)), atol=np.zeros((5, 10)))
+    assert len(runtime.check_patch(10, False, **ok)) == 3
+    st = np.zeros(5, dtype=np.int32)
+    assert len(runtime.check_patch(10, False, status=st, **ok)) == 4
+    bad = [
+        dict(ok, state=np.zeros((5, 10), dtype=np.float32)),       # wrong dtype
+        dict(ok, state=np.zeros((5, 11)), atol=np.zeros((5, 11))),  # wrong column count
+        dict(ok, rtol=np.zeros((4, 10))),                           # rtol rows differ
+        dict(ok, atol=np.zeros((10, 5)).T),                         # not contiguous
+        dict(ok, state=[[0.0] * 10] * 5),                           # not an array
+    ]
+    for b in bad:
+        with pytest.raises(ValueError):
+            runtime.check_patch(10, False, **b)
+    with pytest.raises(ValueError, match="int32"):
+        runtime.check_patch(10, False, status=np.zeros(5, dtype=np.int64), **ok)
+    with pytest.raises(ValueError, match="shape"):
+        runtime.check_patch(10, False, status=np.zeros(4, dtype=np.int32), **ok)
+    with pytest.raises(ValueError, match="isothermal"):
+        runtime.check_patch(10, False, T=np.zeros(5), **ok)
+    dense = dict(state=np.zeros((3, 128)), rtol=np.zeros((3, 128)), atol=np.zeros((3, 128)))
+    with pytest.raises(ValueError, match="temperatures"):
+        runtime.check_patch(128, True, **dense)
+    with pytest.raises(ValueError, match="shape"):
+        runtime.check_patch(128, True, T=np.zeros(4), **dense)
+    assert len(runtime.check_patch(128, True, T=np.zeros(3), **dense)) == 4
