@@ -193,7 +193,7 @@ def test_submit_rejects_oversized_and_unknown(nets):
         rt, at = tolerances(x)
         with pytest.raises(RuntimeError, match="arena holds"):
             srv.submit(PRIM, x, rt, at, 1e10)
-        with pytest.raises(RuntimeError, match="not served"):
+        with pytest.raises(ValueError, match="not served"):   # checked before the C call
             srv.submit(OXY, x, rt, at, 1e10)
         x, _, _ = primordial_ics(64, seed=2)
         rt, at = tolerances(x)
