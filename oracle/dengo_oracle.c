@@ -538,6 +538,38 @@ static void ma_temperature(orc_prim_data *d, const double *u)
     d->dTs_ge = density * mh / (kb * gsum);
 }
 
+/* temperature_from_mass_density (rates_and_rate_tables.C.template:635-668):
+ * species other than ge divided by weight * mh (mass to number density),
+ * T = density ge mh / (kb sum n/(gamma-1)), clamped below at 1 K only.
+ * net NULL: the primordial network. */
+void orc_temperature_from_mass_density(const orc_ma_net *net, const double *input, int nstrip,
+                                       int nchem, double *T)
+{
+    static const int H2G_PRIM[ORC_PRIM_N] = {1, 1, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int n = net ? net->n : ORC_PRIM_N;
+    const int ge = n - 1, de = net ? net->de : DE;
+    const double *Wt = net ? net->W : W;
+    const int *h2 = net ? net->h2gamma : H2G_PRIM;
+    const double kb = 1.3806504e-16, mh = 1.67e-24;
+    const double gamma = 5.e0 / 3.e0, gammaH2 = 7.e0 / 5.e0;
+    if (nchem != n) return;
+    for (int i = 0; i < nstrip; ++i) {
+        double u[ORC_MAXN];
+        for (int s = 0; s < n; ++s) {
+            u[s] = input[(size_t)i * n + s];
+            if (s != ge) u[s] /= Wt[s] * mh;
+        }
+        double density = 0.0, gsum = 0.0;
+        for (int s = 0; s < n; ++s)
+            if (s != ge && s != de) density += Wt[s] * u[s];
+        for (int s = 0; s < n; ++s)
+            if (s != ge) gsum += u[s] / ((h2[s] ? gammaH2 : gamma) - 1.0);
+        double t = density * u[ge] * mh / (kb * gsum);
+        if (t < 1.0) t = 1.0;
+        T[i] = t;
+    }
+}
+
 static double ma_net_change(const orc_rxn *r, int s)
 {
     int up = 0, down = 0;

@@ -141,6 +141,11 @@ double orc_big_evolve(const orc_big_net *net, double dtf, double *dt, const doub
 long long orc_big_eval(const orc_big_net *net, const double *Tcell, const double *u,
                        long long dims, double *f, double *J);
 
+/* temperature_from_mass_density of nstrip cells of mass densities (net NULL:
+ * the primordial network): T clamped below at 1 K, no upper clamp. */
+void orc_temperature_from_mass_density(const orc_ma_net *net, const double *input, int nstrip,
+                                       int nchem, double *T);
+
 /* Mass-action network counterparts of orc_prim_evolve / orc_prim_eval. */
 /* One BE_chem_solve (be; NULL selects orc_BE_chem_solve) of a strip of
  * nstrip cells of the primordial network (net NULL) or a mass-action

@@ -341,6 +341,22 @@ def _strip_lib():
     return lib
 
 
+def temperature_from_mass_density(inp, net=None):
+    """The oracle's temperature_from_mass_density of cells of mass densities
+    inp[n, N] (net: an MANet, or None for the primordial network)."""
+    lib = load()
+    if not getattr(lib, "_tfm_ready", False):
+        lib.orc_temperature_from_mass_density.restype = None
+        lib.orc_temperature_from_mass_density.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int,
+                                                          ctypes.c_int, _dp]
+        lib._tfm_ready = True
+    u = np.ascontiguousarray(inp, dtype=np.float64)
+    T = np.zeros(u.shape[0])
+    netp = None if net is None else ctypes.cast(ctypes.byref(net), ctypes.c_void_p)
+    lib.orc_temperature_from_mass_density(netp, _ptr(u), u.shape[0], u.shape[1], _ptr(T))
+    return T
+
+
 class StripSolver(object):
     """The oracle with the interface of dengo_amd's NetworkSolver strip
     methods (be_chem_solve, ensure_electron_consistency, set_current_z,

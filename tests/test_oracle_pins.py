@@ -187,3 +187,23 @@ def test_mass_input_equals_number_input_with_reference_weights(prim_tables, flag
     assert np.array_equal(a["status"], b["status"])
     assert np.array_equal(a["nsweep"], b["nsweep"])
     assert np.array_equal(a["state"], b["state"] * conv)
+
+
+def test_temperature_from_mass_density_oracle():
+    # rates_and_rate_tables.C.template:635-668: mass densities in, the drawn
+    # temperature out; a lower clamp at 1 K and none above
+    inp, _, T = primordial_ics(400, seed=33, T_range=(2.0, 5e8))
+    got = oracle.temperature_from_mass_density(inp)
+    assert np.allclose(got, T, rtol=1e-12, atol=0)
+    # ideal gas of one species given as a mass density: T is independent of rho
+    for s, w, cv in ((HI, 1.00794, 1.5), (H2I, 2.0, 2.5), (HeI, 4.002602, 1.5)):
+        u = np.zeros((3, 10))
+        u[:, s] = [1e-26, 1e-22, 1e-18]
+        u[:, GE] = 4e12
+        want = w * 4e12 * MH / (KB * cv)
+        assert np.allclose(oracle.temperature_from_mass_density(u), want, rtol=1e-14, atol=0)
+    u = np.zeros((2, 10))
+    u[:, HI] = 1e-24
+    u[:, GE] = [1e-9, 1e22]       # below 1 K and far above the table range
+    got = oracle.temperature_from_mass_density(u)
+    assert got[0] == 1.0 and got[1] > 1e12
