@@ -260,6 +260,9 @@ struct NetHooks {
     int (*prepare_order)(void *data, OrderBufs *b, const long long *order, long long n,
                          void *stream);
     int (*launch)(void *data, const LaunchArgs &a);
+    // like prepare, but allocates and uploads nothing: fails unless the
+    // tables are already resident on the current device
+    int (*resident)(void *data, const dengo_options *opt, int *full_grid);
 };
 NetHooks hooks_primordial();
 NetHooks hooks_oxygen();
@@ -1201,6 +1204,20 @@ struct Host {
         return full_grid(s, oo, full);
     }
 
+    static int hook_resident(void *data, const dengo_options *o, int *full)
+    {
+        Data *d = (Data *)data;
+        dengo_options oo;
+        if (check_options(o, &oo)) return -1;
+        int dev = 0;
+        DENGO_HIP_OK(hipGetDevice(&dev));
+        DevState *s = (DevState *)d->device_state;
+        if (!s || s->device != dev || s->stale || !s->pairs || !s->d_counter)
+            return set_error("the tables of this data struct are not resident on device %d (one "
+                             "evolve or eval call on that device uploads them)", dev);
+        return full_grid(s, oo, full);
+    }
+
     static int hook_prepare_order(void *data, OrderBufs *ob, const long long *order, long long n,
                                   void *stream)
     {
@@ -1229,6 +1246,7 @@ struct Host {
         h.prepare = hook_prepare;
         h.prepare_order = hook_prepare_order;
         h.launch = hook_launch;
+        h.resident = hook_resident;
         return h;
     }
 

@@ -416,6 +416,7 @@ NetHooks dengo::hooks_synthetic()
     h.isothermal = 1;
     h.prepare = hook_prepare;
     h.prepare_order = hook_prepare_order;
+    h.resident = nullptr;
     h.launch = hook_launch;
     return h;
 }

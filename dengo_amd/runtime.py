@@ -168,6 +168,11 @@ SIGNATURES.update({
                                              ctypes.c_longlong, ctypes.POINTER(CellOutputs)]),
     "dengo_graph_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "dengo_graph_destroy": (None, [ctypes.c_void_p]),
+    "dengo_primordial_evolve_device_noalloc": (
+        ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Options), ctypes.c_double,
+                       ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
+                       ctypes.c_void_p, ctypes.c_longlong, ctypes.POINTER(CellOutputs),
+                       _dp, _dp, ctypes.c_void_p]),
     "dengo_host_alloc": (ctypes.c_void_p, [ctypes.c_size_t]),
     "dengo_host_free": (None, [ctypes.c_void_p]),
 })
