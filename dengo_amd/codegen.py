@@ -30,7 +30,7 @@ def _cooling_tables(net):
     return out
 
 
-def render_data_header(net, solver_name):
+def render_data_header(net, solver_name, single=False):
     nT = len(net.T)
     nz = len(net.z)
     energy = net.energy_term.name
@@ -159,6 +159,17 @@ def render_data_header(net, solver_name):
     w("   cell i becomes the free-electron total of cells 0..i (the reference's total")
     w("   runs across the strip). */")
     w("void ensure_electron_consistency_%s(double *input, int nstrip, int nchem);" % n)
+    w("/* temperature_from_mass_density (rates_and_rate_tables.C.template:635-668): the")
+    w("   temperature of nstrip cells given as mass densities (ge in erg/g), clamped below")
+    w("   at 1 K only; computed on the GPU. */")
+    w("void temperature_from_mass_density_%s(double *input, int nstrip, int nchem," % n)
+    w("        double *strip_temperature);")
+    if single:
+        w("/* The same two helpers under the reference's unsuffixed names")
+        w("   (rates_and_rate_tables.h.template:106-107): this library holds one network. */")
+        w("void ensure_electron_consistency(double *input, int nstrip, int nchem);")
+        w("void temperature_from_mass_density(double *input, int nstrip, int nchem,")
+        w("        double *strip_temperature);")
     w("/* Sets data->current_z, the redshift the strip solve uses. */")
     w("void dengo_%s_set_current_z(%s_data *data, double z);" % (n, n))
     w("/* Copies data->Ts (temperatures of the last strip RHS evaluation) for")
@@ -187,7 +198,7 @@ def device_network_kind(net):
     return "prim" if tuple(net.species_list()) == PRIMORDIAL_SPECIES else "ma"
 
 
-def render_instantiation(net, solver_name, kind=None):
+def render_instantiation(net, solver_name, kind=None, single=False):
     """<name>_solver.hip: the kernels of dengo_solver.h instantiated on the
     network's device form, the host traits of <name>_data and the C ABI the
     header declares (dengo/templates/rates_and_rate_tables.C.template
@@ -253,6 +264,9 @@ def render_instantiation(net, solver_name, kind=None):
           "NetHooks hooks_%s() { return NetHost::hooks(); }" % n,
           "}  // namespace dengo",
           ""]
+    if single:
+        L += ["// this library holds one network: the reference's unsuffixed helpers",
+              "DENGO_DEFINE_SINGLE_NETWORK_ABI(%s)" % n, ""]
     return "\n".join(L)
 
 
@@ -315,10 +329,11 @@ def render_solver(net, solver_name, solver_template, ode_solver_source, output_d
         raise ValueError("unknown ode_solver_source %r: the backward-Euler solver is "
                          "'BE_chem_solve_hip.hip'" % ode_solver_source)
     kind = device_network_kind(net)
-    files = [("%s_solver.h" % solver_name, render_data_header(net, solver_name))]
+    files = [("%s_solver.h" % solver_name, render_data_header(net, solver_name, single=True))]
     if kind == "ma":
         files.append(("net_%s.h" % solver_name, render_device_network(net, solver_name)))
-    files.append(("%s_solver.hip" % solver_name, render_instantiation(net, solver_name, kind)))
+    files.append(("%s_solver.hip" % solver_name,
+                  render_instantiation(net, solver_name, kind, single=True)))
     files.append(("%s_run.py" % solver_name, render_run_harness(net, solver_name)))
     written = []
     for name, text in files:

@@ -25,6 +25,10 @@ struct PrimNet {
     using Env = prim::Env;
     static DENGO_D double weight(int k) { return prim::W[k]; }
     static DENGO_HD double free_e(int k) { return prim::FREE_E[k]; }
+    static DENGO_D double temperature(const double (&u)[N], const TableView &tb)
+    {
+        return prim::temperature(u, tb);
+    }
     static DENGO_D bool rhs(const double (&u)[N], double (&f)[N], double (&rs)[NTAB],
                             double &T, const Env &env)
     {

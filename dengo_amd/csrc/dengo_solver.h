@@ -171,6 +171,31 @@ __global__ void __launch_bounds__(64) k_strip_sweep(const double *u, double *u_n
     code[id] = r == SW_FAILED ? 2 + kind : (r == SW_CONVERGED ? 0 : 1);
 }
 
+// temperature_from_mass_density (rates_and_rate_tables.C.template:635-668):
+// species other than ge divided by weight * mh, the network's temperature
+// formula, clamped below at 1 K and not above.
+template <class Net>
+__global__ void __launch_bounds__(64) k_temperature_from_mass(const double *in, long long n,
+                                                              double *T_out)
+{
+    constexpr int N = Net::N;
+    const long long id = (long long)blockIdx.x * 64 + threadIdx.x;
+    if (id >= n) return;
+    double u[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        u[k] = in[id * N + k];
+        if (k != Net::GE) u[k] /= Net::weight(k) * 1.67e-24;
+    }
+    TableView tb;
+    tb.pairs = nullptr;
+    tb.nbins = 0;
+    tb.dbin = tb.idbin = tb.lb = 0.0;
+    tb.T0 = 1.0;
+    tb.T1 = __builtin_huge_val();
+    T_out[id] = Net::temperature(u, tb);
+}
+
 template <int DUMMY>
 __global__ void k_reset_counter(unsigned long long *counter)
 {
@@ -1178,6 +1203,32 @@ struct Host {
         return rv;
     }
 
+    // temperature_from_mass_density (rates_and_rate_tables.C.template:635-668)
+    // for nstrip cells of host memory, computed on the device.
+    static int temperature_from_mass(const double *input, int nstrip, int nchem, double *T)
+    {
+        if (!input || !T) return set_error("null argument");
+        if (nchem != N) return set_error("nchem must be %d", N);
+        if (nstrip <= 0) return 0;
+        double *du = nullptr, *dT = nullptr;
+        const size_t bu = (size_t)nstrip * N * sizeof(double), bT = (size_t)nstrip * sizeof(double);
+        int rc = 0;
+        if (hipMalloc(&du, bu) != hipSuccess || hipMalloc(&dT, bT) != hipSuccess)
+            rc = set_error("hipMalloc failed");
+        if (rc == 0 && hipMemcpy(du, input, bu, hipMemcpyHostToDevice) != hipSuccess)
+            rc = set_error("host-to-device copy failed");
+        if (rc == 0) {
+            hipLaunchKernelGGL(k_temperature_from_mass<Net>, dim3((unsigned)((nstrip + 63) / 64)),
+                               dim3(64), 0, 0, du, (long long)nstrip, dT);
+            if (hipGetLastError() != hipSuccess ||
+                hipMemcpy(T, dT, bT, hipMemcpyDeviceToHost) != hipSuccess)
+                rc = set_error("temperature kernel failed");
+        }
+        if (du) (void)hipFree(du);
+        if (dT) (void)hipFree(dT);
+        return rc;
+    }
+
     // ensure_electron_consistency (rates_and_rate_tables.C.template:604-630):
     // de of cell i is the running total of free electrons over cells 0..i of
     // the strip (the reference does not reset its total per cell).
@@ -1338,6 +1389,11 @@ struct Host {
     {                                                                                          \
         H::electron_consistency(input, nstrip, nchem);                                         \
     }                                                                                          \
+    void temperature_from_mass_density_##n(double *input, int nstrip, int nchem, double *T)    \
+    {                                                                                          \
+        if (H::temperature_from_mass(input, nstrip, nchem, T) != 0)                            \
+            fprintf(stderr, "temperature_from_mass_density_" #n ": %s\n", dengo_last_error()); \
+    }                                                                                          \
     void dengo_##n##_set_current_z(n##_data *d, double z)                                      \
     {                                                                                          \
         if (d) d->current_z = z;                                                               \
@@ -1388,5 +1444,20 @@ struct Host {
     void n##_calculate_temperature(n##_data *d, double *in, int ns, int nc)                    \
     {                                                                                          \
         H::calculate_temperature(d, in, ns, nc);                                               \
+    }                                                                                          \
+    }
+
+// The reference's unsuffixed helpers (rates_and_rate_tables.h.template:106-107),
+// exported only by a library that holds a single network (the solver
+// libraries write_solver emits): their names carry no network.
+#define DENGO_DEFINE_SINGLE_NETWORK_ABI(n)                                                     \
+    extern "C" {                                                                               \
+    void ensure_electron_consistency(double *input, int nstrip, int nchem)                     \
+    {                                                                                          \
+        ensure_electron_consistency_##n(input, nstrip, nchem);                                 \
+    }                                                                                          \
+    void temperature_from_mass_density(double *input, int nstrip, int nchem, double *T)        \
+    {                                                                                          \
+        temperature_from_mass_density_##n(input, nstrip, nchem, T);                            \
     }                                                                                          \
     }

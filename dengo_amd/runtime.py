@@ -104,6 +104,7 @@ def _network_signatures(n):
         "dengo_%s_be_chem_solve" % n: (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_double, _dp,
                                                       _dp, ctypes.c_int, ctypes.c_int, _dp]),
         "ensure_electron_consistency_%s" % n: (None, [_dp, ctypes.c_int, ctypes.c_int]),
+        "temperature_from_mass_density_%s" % n: (None, [_dp, ctypes.c_int, ctypes.c_int, _dp]),
         "dengo_%s_set_current_z" % n: (None, [ctypes.c_void_p, ctypes.c_double]),
         "dengo_%s_strip_temperatures" % n: (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int]),
         "%s_main" % n: (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p]),
@@ -129,7 +130,15 @@ COMMON_SIGNATURES = {
 }
 
 
-# Every symbol include/*.h declares, with its ctypes signature.
+# The reference's unsuffixed helpers, exported only by single-network solver
+# libraries (include/dengo_single_network.h).
+SINGLE_NETWORK_SIGNATURES = {
+    "ensure_electron_consistency": (None, [_dp, ctypes.c_int, ctypes.c_int]),
+    "temperature_from_mass_density": (None, [_dp, ctypes.c_int, ctypes.c_int, _dp]),
+}
+
+# Every symbol include/*.h declares for libdengo_hip.so, with its ctypes
+# signature.
 SIGNATURES = dict(COMMON_SIGNATURES)
 for _n in NETWORKS:
     SIGNATURES.update(_network_signatures(_n))
@@ -210,6 +219,7 @@ def load_solver_library(path, name):
     lib = ctypes.CDLL(path)
     sigs = dict(COMMON_SIGNATURES)
     sigs.update(_network_signatures(name))
+    sigs.update(SINGLE_NETWORK_SIGNATURES)
     for fn_name, (res, args) in sigs.items():
         fn = getattr(lib, fn_name)
         fn.restype = res
@@ -389,6 +399,16 @@ class NetworkSolver(object):
         if not (u.dtype == np.float64 and u.flags["C_CONTIGUOUS"]):
             raise ValueError("expected a C-contiguous float64 array")
         self._fn("ensure_electron_consistency_%s")(_p(u), u.shape[0], u.shape[1])
+
+    def temperature_from_mass_density(self, u):
+        """temperature_from_mass_density of cells of mass densities u[n, N]
+        (rates_and_rate_tables.C.template:635-668), on the GPU."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        if u.ndim != 2 or u.shape[1] != self.N:
+            raise ValueError("expected an array of shape [n, %d]" % self.N)
+        T = np.full(u.shape[0], np.nan)
+        self._fn("temperature_from_mass_density_%s")(_p(u), u.shape[0], u.shape[1], _p(T))
+        return T
 
     def set_current_z(self, z):
         self._fn("dengo_%s_set_current_z")(self.data, z)

@@ -209,6 +209,11 @@ int dengo_oxygen_be_chem_solve(oxygen_data *data, double *u, double dt, double *
    cell i becomes the free-electron total of cells 0..i (the reference's total
    runs across the strip). */
 void ensure_electron_consistency_oxygen(double *input, int nstrip, int nchem);
+/* temperature_from_mass_density (rates_and_rate_tables.C.template:635-668): the
+   temperature of nstrip cells given as mass densities (ge in erg/g), clamped below
+   at 1 K only; computed on the GPU. */
+void temperature_from_mass_density_oxygen(double *input, int nstrip, int nchem,
+        double *strip_temperature);
 /* Sets data->current_z, the redshift the strip solve uses. */
 void dengo_oxygen_set_current_z(oxygen_data *data, double z);
 /* Copies data->Ts (temperatures of the last strip RHS evaluation) for

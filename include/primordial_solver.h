@@ -289,6 +289,11 @@ int dengo_primordial_be_chem_solve(primordial_data *data, double *u, double dt, 
    cell i becomes the free-electron total of cells 0..i (the reference's total
    runs across the strip). */
 void ensure_electron_consistency_primordial(double *input, int nstrip, int nchem);
+/* temperature_from_mass_density (rates_and_rate_tables.C.template:635-668): the
+   temperature of nstrip cells given as mass densities (ge in erg/g), clamped below
+   at 1 K only; computed on the GPU. */
+void temperature_from_mass_density_primordial(double *input, int nstrip, int nchem,
+        double *strip_temperature);
 /* Sets data->current_z, the redshift the strip solve uses. */
 void dengo_primordial_set_current_z(primordial_data *data, double z);
 /* Copies data->Ts (temperatures of the last strip RHS evaluation) for

@@ -185,6 +185,16 @@ int dengo_carbon_be_chem_solve(carbon_data *data, double *u, double dt, double *
    cell i becomes the free-electron total of cells 0..i (the reference's total
    runs across the strip). */
 void ensure_electron_consistency_carbon(double *input, int nstrip, int nchem);
+/* temperature_from_mass_density (rates_and_rate_tables.C.template:635-668): the
+   temperature of nstrip cells given as mass densities (ge in erg/g), clamped below
+   at 1 K only; computed on the GPU. */
+void temperature_from_mass_density_carbon(double *input, int nstrip, int nchem,
+        double *strip_temperature);
+/* The same two helpers under the reference's unsuffixed names
+   (rates_and_rate_tables.h.template:106-107): this library holds one network. */
+void ensure_electron_consistency(double *input, int nstrip, int nchem);
+void temperature_from_mass_density(double *input, int nstrip, int nchem,
+        double *strip_temperature);
 /* Sets data->current_z, the redshift the strip solve uses. */
 void dengo_carbon_set_current_z(carbon_data *data, double z);
 /* Copies data->Ts (temperatures of the last strip RHS evaluation) for

@@ -47,3 +47,6 @@ DENGO_DEFINE_C_ABI(carbon, NetHost)
 namespace dengo {
 NetHooks hooks_carbon() { return NetHost::hooks(); }
 }  // namespace dengo
+
+// this library holds one network: the reference's unsuffixed helpers
+DENGO_DEFINE_SINGLE_NETWORK_ABI(carbon)

@@ -1,0 +1,78 @@
+"""The reference's host helpers through the C ABI on the GPU:
+temperature_from_mass_density_<n> (rates_and_rate_tables.C.template:635-668)
+bitwise against the oracle for the primordial and oxygen networks, and the
+unsuffixed ensure_electron_consistency / temperature_from_mass_density
+(rates_and_rate_tables.h.template:106-107) of a single-network solver library
+(the generated carbon network) against the oracle and the suffixed forms."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from dengo_amd import runtime
+from dengo_amd.ics import ion_ics, oxygen_ics, primordial_ics
+from dengo_amd.networks import oxygen_tables, primordial_tables
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CARBON_LIB = os.path.join(ROOT, "tests", "generated", "carbon", "libcarbon_solver.so")
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def _edge_cells(x, ge):
+    """Append cells whose temperature falls below 1 K and far above the
+    table range (the helper clamps only below)."""
+    e = np.repeat(x[:2], 2, axis=0)
+    e[0, ge] *= 1e-12
+    e[1, ge] *= 1e6
+    e[2, ge] *= 1e-30
+    e[3, ge] *= 1e9
+    return np.ascontiguousarray(np.concatenate([x, e]))
+
+
+def test_temperature_from_mass_density_primordial():
+    print("boundary: temperature_from_mass_density_primordial", flush=True)
+    s = runtime.PrimordialSolver(primordial_tables()[1])
+    inp = _edge_cells(primordial_ics(2000, seed=91)[0], 9)
+    got = s.temperature_from_mass_density(inp)
+    want = oracle.temperature_from_mass_density(inp)
+    assert np.array_equal(got, want)
+    assert got.min() == 1.0 and got.max() > 1e9
+    s.close()
+
+
+def test_temperature_from_mass_density_oxygen():
+    print("boundary: temperature_from_mass_density_oxygen", flush=True)
+    s = runtime.OxygenSolver(oxygen_tables()[1])
+    inp = _edge_cells(oxygen_ics(1500, seed=92)[0], 10)
+    got = s.temperature_from_mass_density(inp)
+    want = oracle.temperature_from_mass_density(inp, oracle.ion_chain_net(8, 15.9994))
+    assert np.array_equal(got, want)
+    s.close()
+
+
+def test_unsuffixed_helpers_of_a_single_network_library():
+    print("boundary: unsuffixed helpers of the carbon library", flush=True)
+    lib = runtime.load_solver_library(CARBON_LIB, "carbon")
+    x = ion_ics(700, 93, 7, 12.0107)[0]
+    T = np.zeros(x.shape[0])
+    lib.temperature_from_mass_density(x.ctypes.data_as(_dp), x.shape[0], x.shape[1],
+                                      T.ctypes.data_as(_dp))
+    assert np.array_equal(T, oracle.temperature_from_mass_density(
+        x, oracle.ion_chain_net(6, 12.0107)))
+    T2 = np.zeros_like(T)
+    lib.temperature_from_mass_density_carbon(x.ctypes.data_as(_dp), x.shape[0], x.shape[1],
+                                             T2.ctypes.data_as(_dp))
+    assert np.array_equal(T, T2)
+    a, b = x.copy(), x.copy()
+    lib.ensure_electron_consistency(a.ctypes.data_as(_dp), a.shape[0], a.shape[1])
+    lib.ensure_electron_consistency_carbon(b.ctypes.data_as(_dp), b.shape[0], b.shape[1])
+    assert np.array_equal(a, b)
+    c = x.copy()
+    oracle.load()
+    oracle._strip_lib().orc_strip_electrons(
+        ctypes.cast(ctypes.byref(oracle.ion_chain_net(6, 12.0107)), ctypes.c_void_p),
+        c.ctypes.data_as(_dp), c.shape[0])
+    assert np.array_equal(a, c)
