@@ -72,17 +72,33 @@ def build_solver(output_dir, solver_name, verbose=False):
 GENERATED = os.path.join(ROOT, "tests", "generated")
 
 
-def build_generated_examples():
-    """Writes (ChemicalNetwork.write_solver) and builds the generated solvers
-    the tests load: carbon, an ion-by-ion network with no hand-written code
-    in this package (tests/generated/carbon)."""
+def generated_example_networks():
+    """(name, network) of the generated solvers the tests load:
+    carbon, an ion-by-ion network (mass-action descriptor); primgen, the full
+    primordial network through the generator instead of the hand-written
+    device code; primphoto, the primordial species with a reduced reaction
+    and cooling set plus photoionisation and photoheating on redshift
+    tables."""
     if ROOT not in sys.path:
         sys.path.insert(0, ROOT)
-    from dengo_amd.networks import ion_network
-    out = os.path.join(GENERATED, "carbon")
-    net = ion_network("C", seed=9)
-    net.write_solver("carbon", output_dir=out, write_h5=False)
-    return [build_solver(out, "carbon")]
+    import numpy as np
+    from dengo_amd.networks import ion_network, primordial_network, primordial_photo_network
+    with np.errstate(all="ignore"):
+        return [("carbon", ion_network("C", seed=9)), ("primgen", primordial_network()),
+                ("primphoto", primordial_photo_network())]
+
+
+def build_generated_examples():
+    """Writes (ChemicalNetwork.write_solver) and builds the generated solvers
+    the tests load into tests/generated/<name>."""
+    libs = []
+    for name, net in generated_example_networks():
+        out = os.path.join(GENERATED, name)
+        import numpy as np
+        with np.errstate(all="ignore"):
+            net.write_solver(name, output_dir=out, write_h5=False)
+        libs.append(build_solver(out, name))
+    return libs
 
 
 if __name__ == "__main__":

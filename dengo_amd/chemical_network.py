@@ -126,14 +126,46 @@ class ChemicalNetwork(object):
                 names.append("%s_%s" % (ca.name, t))
         return names
 
+    def is_z_table(self, table):
+        """Whether the table of this name is interpolated in redshift rather
+        than temperature: reactions whose name contains "pi" and every table of
+        a cooling action whose name contains "ph"
+        (rates_and_rate_tables.C.template:402-449; h.template:57-80 sizes them
+        by the z grid)."""
+        if table in self.reactions:
+            return "pi" in table
+        for ca in self.cooling_actions.values():
+            if any("%s_%s" % (ca.name, t) == table for t in ca.tables):
+                return "ph" in ca.name
+        raise KeyError("no table %s in the network" % table)
+
+    def z_table_flags(self):
+        """is_z_table for every table, in struct order."""
+        return [self.is_z_table(t) for t in self.table_names()]
+
     def table_block(self):
-        """All tables evaluated on ``self.T`` as a [ntables, len(T)] FP64 array
-        (the values write_solver stores, dengo/chemical_network.py:310-318)."""
-        rows = [np.asarray(r.coeff_fn(self), dtype=np.float64)
-                for r in self.ordered_reactions()]
+        """All tables as a [ntables, len(T)] FP64 array (the values
+        write_solver stores, dengo/chemical_network.py:310-318): temperature
+        tables evaluated on ``self.T``; a redshift table holds its len(z)
+        values on ``self.z`` followed by zeros."""
+        vals = [r.coeff_fn(self) for r in self.ordered_reactions()]
         for ca in self.ordered_cooling():
             for t in sorted(ca.tables, key=str.lower):
-                rows.append(np.asarray(ca.tables[t](self), dtype=np.float64))
+                vals.append(ca.tables[t](self))
+        nT, nz = len(self.T), len(self.z)
+        rows = []
+        for name, z, v in zip(self.table_names(), self.z_table_flags(), vals):
+            v = np.asarray(v, dtype=np.float64).ravel()
+            want = nz if z else nT
+            if v.size != want:
+                raise ValueError("table %s has %d values, the %s grid %d" % (
+                    name, v.size, "redshift" if z else "temperature", want))
+            if z:
+                if nz > nT:
+                    raise ValueError("redshift grid (%d) longer than the temperature grid (%d)"
+                                     % (nz, nT))
+                v = np.concatenate([v, np.zeros(nT - nz)])
+            rows.append(v)
         return np.ascontiguousarray(np.stack(rows))
 
     def __iter__(self):
@@ -248,6 +280,8 @@ class ChemicalNetwork(object):
             "T_bounds": [float(self.T_bounds[0]), float(self.T_bounds[1])],
             "z_bounds": [float(self.z_bounds[0]), float(self.z_bounds[1])],
             "n_z": int(len(self.z)),
+            "table_lengths": [int(len(self.z)) if z else int(block.shape[1])
+                              for z in self.z_table_flags()],
             "species": self.species_list(),
             "weights": [float(s.weight) for s in self.ordered_species()],
             "dtype": "<f8",

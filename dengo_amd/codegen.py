@@ -54,6 +54,10 @@ def render_data_header(net, solver_name, single=False):
     w("#define %s_T_LO %r" % (solver_name.upper(), float(net.T_bounds[0])))
     w("#define %s_T_HI %r" % (solver_name.upper(), float(net.T_bounds[1])))
     w("#define %s_STOP_TIME %r" % (solver_name.upper(), float(net.stop_time)))
+    w("/* Redshift grid of the pi/ph tables (rates_and_rate_tables.C.template:31-36). */")
+    w("#define %s_NZ %d" % (solver_name.upper(), nz))
+    w("#define %s_Z_LO %r" % (solver_name.upper(), float(net.z_bounds[0])))
+    w("#define %s_Z_HI %r" % (solver_name.upper(), float(net.z_bounds[1])))
     w("")
     w("typedef struct %s_data_ {" % solver_name)
     w("    double dbin;")
@@ -70,12 +74,12 @@ def render_data_header(net, solver_name, single=False):
     for f in ("Ts", "Tdef", "dT", "logTs", "invTs", "dTs_%s" % energy):
         w("    double %s[MAX_NCELLS];" % f)
     for name in _reaction_names(net):
-        n = nz if "pi" in name else nT
+        n = nz if net.is_z_table(name) else nT
         w("    double r_%s[%d];" % (name, n))
         w("    double rs_%s[MAX_NCELLS];" % name)
         w("    double drs_%s[MAX_NCELLS];" % name)
     for act, tab in _cooling_tables(net):
-        n = nz if "ph" in act else nT
+        n = nz if net.is_z_table("%s_%s" % (act, tab)) else nT
         w("    double c_%s_%s[%d];" % (act, tab, n))
         w("    double cs_%s_%s[MAX_NCELLS];" % (act, tab))
         w("    double dcs_%s_%s[MAX_NCELLS];" % (act, tab))
@@ -93,6 +97,9 @@ def render_data_header(net, solver_name, single=False):
     w("#define %s_TABLE_FIELDS(X) \\" % solver_name.upper())
     for i, fld in enumerate(fields):
         w("    X(%s)%s" % (fld, " \\" if i + 1 < len(fields) else ""))
+    w("/* Values per table in struct order: the T grid, or the z grid for a redshift table. */")
+    w("#define %s_TABLE_LENGTHS {%s}" % (solver_name.upper(), ", ".join(
+        str(nz if z else nT) for z in net.z_table_flags())))
     w("")
     w("#include <stddef.h>")
     w("")
@@ -192,10 +199,19 @@ PRIMORDIAL_SPECIES = ("H2I", "H2II", "HI", "HII", "HM", "HeI", "HeII", "HeIII", 
 
 
 def device_network_kind(net):
-    """'prim' for the primordial species set (the hand-written device network
-    PrimNet, whose cooling terms are not monomials), 'ma' for networks the
-    generic mass-action device network MANet evaluates from a descriptor."""
-    return "prim" if tuple(net.species_list()) == PRIMORDIAL_SPECIES else "ma"
+    """How write_solver emits a network's device code: 'ma' for a mass-action
+    network whose cooling terms are all  coef * table * species * de^p  and
+    which has no redshift tables (a descriptor for the generic device network
+    MANet, ma_net.h), 'gen' for every other network (RHS and Jacobian printed
+    from the sympy forms, gen_net.h).  The hand-written primordial device
+    network ('prim', PrimNet) serves only the built-in primordial library."""
+    if any(net.z_table_flags()):
+        return "gen"
+    try:
+        _cooling_monomials(net)
+    except ValueError:
+        return "gen"
+    return "ma"
 
 
 def render_instantiation(net, solver_name, kind=None, single=False):
@@ -211,11 +227,16 @@ def render_instantiation(net, solver_name, kind=None, single=False):
         net_type = "dengo::PrimNet"
         includes = ['#include "dengo_solver.h"', '#include "%s_solver.h"' % n]
         what = "the hand-written device network (dengo_device.h)"
-    else:
+    elif kind == "ma":
         net_type = "dengo::MANet<dengo::net_%s::Desc>" % n
         includes = ['#include "dengo_solver.h"', '#include "net_%s.h"' % n,
                     '#include "%s_solver.h"' % n]
         what = "the generated descriptor net_%s.h" % n
+    else:
+        net_type = "dengo::GenNet<dengo::net_%s::Gen>" % n
+        includes = ['#include "dengo_solver.h"', '#include "net_%s.h"' % n,
+                    '#include "%s_solver.h"' % n]
+        what = "the generated network net_%s.h (RHS and Jacobian from sympy)" % n
     L = ["// Generated by dengo_amd.codegen: do not edit.",
          "// C ABI of the %s network (%s): kernels from" % (n, species),
          "// dengo_solver.h instantiated on %s, host struct" % what,
@@ -231,6 +252,14 @@ def render_instantiation(net, solver_name, kind=None, single=False):
           "    static constexpr double T_LO = %s_T_LO;" % N,
           "    static constexpr double T_HI = %s_T_HI;" % N,
           "    static constexpr double STOP_TIME = %s_STOP_TIME;" % N,
+          "    static constexpr int NZ = %s_NZ;" % N,
+          "    static constexpr double Z_LO = %s_Z_LO;" % N,
+          "    static constexpr double Z_HI = %s_Z_HI;" % N,
+          "    static const int *table_lengths()",
+          "    {",
+          "        static const int l[] = %s_TABLE_LENGTHS;" % N,
+          "        return l;",
+          "    }",
           "    static const size_t *table_offsets()",
           "    {",
           "        static const size_t o[] = {",
@@ -332,6 +361,8 @@ def render_solver(net, solver_name, solver_template, ode_solver_source, output_d
     files = [("%s_solver.h" % solver_name, render_data_header(net, solver_name, single=True))]
     if kind == "ma":
         files.append(("net_%s.h" % solver_name, render_device_network(net, solver_name)))
+    else:
+        files.append(("net_%s.h" % solver_name, render_gen_network(net, solver_name)))
     files.append(("%s_solver.hip" % solver_name,
                   render_instantiation(net, solver_name, kind, single=True)))
     files.append(("%s_run.py" % solver_name, render_run_harness(net, solver_name)))
@@ -373,6 +404,26 @@ def _monomial(term, net, table_syms, de_name="de"):
     return float(coef), table, species, depow
 
 
+def _cooling_monomials(net):
+    """(table index, species index or -1, de power, coefficient) of every
+    cooling term; raises ValueError unless each is a monomial
+    coef * table * [species] * de^p with p <= 2."""
+    import sympy
+    species = net.species_list()
+    idx = {s: i for i, s in enumerate(species)}
+    tidx = {t: i for i, t in enumerate(net.table_names())}
+    cool = []
+    for ca in net.ordered_cooling():
+        syms = {"%s_%s[i]" % (ca.name, t): tidx["%s_%s" % (ca.name, t)] for t in ca.tables}
+        eq = sympy.expand(ca.equation)
+        for term in sympy.Add.make_args(eq):
+            if term == 0:
+                continue
+            coef, t, sp, p = _monomial(term, net, syms)
+            cool.append((t, idx[sp] if sp else -1, p, coef))
+    return cool
+
+
 def render_device_network(net, name):
     """C++ descriptor of a mass-action network with cooling terms of the form
     coef * table * species * de^p (ion-by-ion networks), consumed by the
@@ -395,15 +446,7 @@ def render_device_network(net, name):
         if len(slots) > 3 or len(prods) > 4:
             raise ValueError("reaction %s: at most 3 reactants and 4 products" % r.name)
         rx.append((tidx[r.name], slots, sorted(prods.items())))
-    cool = []
-    for ca in net.ordered_cooling():
-        syms = {"%s_%s[i]" % (ca.name, t): tidx["%s_%s" % (ca.name, t)] for t in ca.tables}
-        eq = sympy.expand(ca.equation)
-        for term in sympy.Add.make_args(eq):
-            if term == 0:
-                continue
-            coef, t, sp, p = _monomial(term, net, syms)
-            cool.append((t, idx[sp] if sp else -1, p, coef))
+    cool = _cooling_monomials(net)
     W = [float(s.weight) for s in net.ordered_species()]
     FE = [float(getattr(s, "free_electrons", 0.0)) for s in net.ordered_species()]
     H2 = ["true" if s in ("H2I", "H2II") else "false" for s in species]
@@ -439,4 +482,168 @@ def render_device_network(net, name):
     for t, sp, p, c in cool:
         L.append("        {%d, %d, %d, %r}," % (t, sp, p, c))
     L += ["    };", "};", "}  // namespace net_%s" % name, "}  // namespace dengo", ""]
+    return "\n".join(L)
+
+
+# ---------------------------------------------------------------------------
+# Generated device network (gen_net.h): RHS and Jacobian printed from sympy
+# ---------------------------------------------------------------------------
+def _device_printer(names):
+    """C printer for device arithmetic: symbols renamed through `names`,
+    small integer powers as products (x**2 -> (x*x), x**-1 -> (1.0/x)), other
+    powers through pow/sqrt, floats with 17 significant digits."""
+    import sympy
+    from sympy.printing.c import C99CodePrinter
+    from sympy.printing.precedence import PRECEDENCE
+
+    class Printer(C99CodePrinter):
+        def _print_Symbol(self, expr):
+            return names.get(expr.name, expr.name)
+
+        def _print_Pow(self, expr):
+            b, e = expr.base, expr.exp
+            if e.is_Float and float(e) == int(float(e)):
+                e = sympy.Integer(int(float(e)))
+            if e.is_Integer and 1 <= abs(int(e)) <= 4:
+                k = abs(int(e))
+                base = self.parenthesize(b, PRECEDENCE["Mul"])
+                prod = "*".join([base] * k)
+                return "(%s)" % prod if int(e) > 0 else "(1.0/(%s))" % prod
+            if e == sympy.S.Half:
+                return "sqrt(%s)" % self._print(b)
+            return "pow(%s, %s)" % (self._print(b), self._print(sympy.Float(float(e))
+                                                              if e.is_Number else e))
+    return Printer()
+
+
+def gen_network_forms(net):
+    """The symbolic forms the generated network evaluates: (species symbols in
+    solver order, table symbols in struct order, rhs list [N] -- species_total
+    for the species rows, the cooling sum for the energy row --, Jacobian
+    dict {(row, col): expr} of the nonzero entries, columns other than the
+    energy)."""
+    import sympy
+    sp = net.ordered_species()
+    syms = [s.symbol for s in sp]
+    ge = net.energy_term
+    rates = [r.coeff_sym for r in net.ordered_reactions()]
+    for ca in net.ordered_cooling():
+        ca.equation     # fills ca.table_symbols
+        for t in sorted(ca.tables, key=str.lower):
+            rates.append(ca.table_symbols[t])
+    rhs = [net.cooling_total() if s == ge else net.species_total(s) for s in sp]
+    rhs = [sympy.sympify(e) for e in rhs]
+    jac = {}
+    for r, e in enumerate(rhs):
+        for c, x in enumerate(syms):
+            if sp[c] == ge:
+                continue            # d/dge is identically zero (the reference's quirk)
+            d = sympy.diff(e, x)
+            if d != 0:
+                jac[(r, c)] = d
+    return syms, rates, rhs, jac
+
+
+def render_gen_network(net, name):
+    """net_<name>.h: the network's RHS and Jacobian as straight-line C++
+    printed from the sympy forms after common-subexpression elimination, with
+    the species layout, redshift-table slots and the species block's
+    structural nonzeros, for GenNet (dengo_amd/csrc/gen_net.h)."""
+    import sympy
+    species = net.species_list()
+    n = len(species)
+    assert species[-1] == net.energy_term.name, "energy must sort last"
+    syms, rates, rhs, jac = gen_network_forms(net)
+    names = {str(x): "u[%d]" % i for i, x in enumerate(syms)}
+    names.update({str(r): "rs[%d]" % t for t, r in enumerate(rates)})
+    names["T"] = "T"
+    names["z"] = "z"
+    known = set(names)
+    for e in list(rhs) + list(jac.values()):
+        for a in e.free_symbols:
+            if a.name not in known:
+                raise ValueError("network %s: the symbol %s in its equations is not a species, "
+                                 "table, T or z" % (name, a.name))
+    zflags = net.z_table_flags()
+    zslot, k = [], 0
+    for f in zflags:
+        zslot.append(k if f else -1)
+        k += 1 if f else 0
+    if k > 16:
+        raise ValueError("network %s has %d redshift tables; at most 16" % (name, k))
+    de = species.index("de") if "de" in species else -1
+    if de < 0:
+        raise ValueError("network %s has no electron species de" % name)
+    pr = _device_printer(names)
+
+    def body(exprs, targets):
+        tmp = sympy.numbered_symbols("c")
+        reps, red = sympy.cse(exprs, symbols=tmp, optimizations="basic")
+        out = []
+        for sym, e in reps:
+            names[sym.name] = sym.name
+            out.append("        const double %s = %s;" % (sym.name, pr.doprint(e)))
+        for tgt, e in zip(targets, red):
+            out.append("        %s = %s;" % (tgt, pr.doprint(e)))
+        return out
+
+    W = [float(s.weight) for s in net.ordered_species()]
+    FE = [float(getattr(s, "free_electrons", 0.0)) for s in net.ordered_species()]
+    H2 = ["true" if s in ("H2I", "H2II") else "false" for s in species]
+    m = n - 1
+    jnz = [["true" if (i, j) in jac else "false" for j in range(m)] for i in range(m)]
+
+    def arr(v):
+        return "{" + ", ".join(repr(x) for x in v) + "}"
+
+    L = ["// Generated by dengo_amd.codegen from the %s network: do not edit." % name,
+         "// Species (solver order): %s" % " ".join(species),
+         "// Tables (struct order): %s" % " ".join(net.table_names()),
+         "// rhs: species_total(s) per species row, the cooling sum in the energy row",
+         "// (dengo/chemical_network.py:118-151); jac: their derivatives by each species,",
+         "// column-major, energy column zero (:153-168).  Common subexpressions by sympy.cse.",
+         "#pragma once",
+         "#include <math.h>",
+         "",
+         "#ifndef DENGO_GEN_FN",
+         '#error "define DENGO_GEN_FN (the function qualifiers) before including this header"',
+         "#endif",
+         "",
+         "namespace dengo {",
+         "namespace net_%s {" % name,
+         "struct Gen {",
+         "    static constexpr int N = %d, NTAB = %d, DE = %d, GE = %d;" % (
+             n, len(rates), de, n - 1),
+         "    static constexpr double W[N] = %s;" % arr(W),
+         "    static constexpr double FREE_E[N] = %s;" % arr(FE),
+         "    static constexpr bool H2G[N] = {%s};" % ", ".join(H2),
+         "    // redshift-table slot of each table (-1: a temperature table)",
+         "    static constexpr int ZTAB[NTAB] = {%s};" % ", ".join(map(str, zslot)),
+         "    // structural nonzeros of the species block of the Jacobian",
+         "    static constexpr bool JNZ[N - 1][N - 1] = {"]
+    for row in jnz:
+        L.append("        {%s}," % ", ".join(row))
+    L += ["    };",
+          "",
+          "    static DENGO_GEN_FN void rhs(const double (&u)[N], const double (&rs)[NTAB], "
+          "double T,",
+          "                                 double z, double (&f)[N])",
+          "    {",
+          "        (void)T;",
+          "        (void)z;"]
+    L += body(rhs, ["f[%d]" % i for i in range(n)])
+    L += ["    }",
+          "",
+          "    static DENGO_GEN_FN void jac(const double (&u)[N], const double (&rs)[NTAB], "
+          "double T,",
+          "                                 double z, double (&J)[N * N])",
+          "    {",
+          "        (void)T;",
+          "        (void)z;"]
+    keys = sorted(jac, key=lambda rc: (rc[1], rc[0]))
+    zero = [(r, c) for c in range(n) for r in range(n) if (r, c) not in jac]
+    L += body([jac[k] for k in keys], ["J[%d]" % (c * n + r) for r, c in keys])
+    for r, c in zero:
+        L.append("        J[%d] = 0.0;" % (c * n + r))
+    L += ["    }", "};", "}  // namespace net_%s" % name, "}  // namespace dengo", ""]
     return "\n".join(L)

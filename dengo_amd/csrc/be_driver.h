@@ -25,6 +25,8 @@ struct PrimNet {
     using Env = prim::Env;
     static DENGO_D double weight(int k) { return prim::W[k]; }
     static DENGO_HD double free_e(int k) { return prim::FREE_E[k]; }
+    // redshift table slot of table t (-1: a temperature table)
+    static constexpr int ztab(int) { return -1; }
     static DENGO_D double temperature(const double (&u)[N], const TableView &tb)
     {
         return prim::temperature(u, tb);

@@ -295,9 +295,18 @@ DENGO_D void cooling_grad(const double (&c)[NTAB], const double (&u)[N], double 
     g[DE] += -c[REHII] * u[HII];
 }
 
+// Redshift tables (reaction names containing "pi", cooling actions whose
+// name contains "ph") per network, at most this many.
+constexpr int kMaxZTables = 16;
+
 struct Env {
     TableView tb;
     double zp1, comp4;
+    double z;                   // the call's redshift
+    // value of each redshift table at z, interpolated once per call on the
+    // host (rates_and_rate_tables.C.template:384-401, the pi/ph branches of
+    // :402-449): 0 when z lies outside the tables' range
+    double zr[kMaxZTables];
 };
 
 // calculate_rhs (rates_and_rate_tables.C.template:460-522) for one cell.

@@ -15,6 +15,14 @@ struct Traits {
     static constexpr double T_LO = OXYGEN_T_LO;
     static constexpr double T_HI = OXYGEN_T_HI;
     static constexpr double STOP_TIME = OXYGEN_STOP_TIME;
+    static constexpr int NZ = OXYGEN_NZ;
+    static constexpr double Z_LO = OXYGEN_Z_LO;
+    static constexpr double Z_HI = OXYGEN_Z_HI;
+    static const int *table_lengths()
+    {
+        static const int l[] = OXYGEN_TABLE_LENGTHS;
+        return l;
+    }
     static const size_t *table_offsets()
     {
         static const size_t o[] = {

@@ -14,6 +14,14 @@ struct Traits {
     static constexpr double T_LO = PRIMORDIAL_T_LO;
     static constexpr double T_HI = PRIMORDIAL_T_HI;
     static constexpr double STOP_TIME = PRIMORDIAL_STOP_TIME;
+    static constexpr int NZ = PRIMORDIAL_NZ;
+    static constexpr double Z_LO = PRIMORDIAL_Z_LO;
+    static constexpr double Z_HI = PRIMORDIAL_Z_HI;
+    static const int *table_lengths()
+    {
+        static const int l[] = PRIMORDIAL_TABLE_LENGTHS;
+        return l;
+    }
     static const size_t *table_offsets()
     {
         static const size_t o[] = {

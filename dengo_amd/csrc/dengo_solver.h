@@ -27,6 +27,7 @@
 #include "be_driver.h"
 #include "be_fast.h"
 #include "ma_net.h"
+#include "gen_net.h"
 #include "dengo_h5.h"
 #include "dengo_types.h"
 
@@ -456,7 +457,8 @@ inline int check_options(const dengo_options *opt, dengo_options *o)
 }
 
 // Traits (per network): Data (struct), NSPEC, NTABLES, NT, T_LO/T_HI (the
-// temperature grid), STOP_TIME (<n>_main's dtf), table_offsets(),
+// temperature grid), NZ, Z_LO/Z_HI (the redshift grid), STOP_TIME (<n>_main's
+// dtf), table_offsets(), table_lengths() (NT, or NZ for a redshift table),
 // table_names() (struct field names), field_names().
 template <class Net, class Tr>
 struct Host {
@@ -476,6 +478,8 @@ struct Host {
         return (const double *)((const char *)d + Tr::table_offsets()[t]);
     }
 
+    static int table_len(int t) { return Tr::table_lengths()[t]; }
+
     static void set_grid(Data *d, double T0, double T1)
     {
         // rates_and_rate_tables.C.template:25-36
@@ -484,11 +488,33 @@ struct Host {
         d->nbins = Tr::NT - 1;
         d->dbin = (log(d->bounds[1]) - log(d->bounds[0])) / d->nbins;
         d->idbin = (double)(1.0L / d->dbin);
-        d->z_bounds[0] = 0.0;
-        d->z_bounds[1] = 0.0;
-        d->n_zbins = 0;
-        d->d_zbin = 0.0;
-        d->id_zbin = 0.0;
+        d->z_bounds[0] = Tr::Z_LO;
+        d->z_bounds[1] = Tr::Z_HI;
+        d->n_zbins = Tr::NZ - 1;
+        if (d->n_zbins > 0) {
+            d->d_zbin = (log(d->z_bounds[1] + 1.0) - log(d->z_bounds[0] + 1.0)) / d->n_zbins;
+            d->id_zbin = (double)(1.0L / d->d_zbin);
+        } else {
+            d->d_zbin = 0.0;
+            d->id_zbin = 0.0;
+        }
+    }
+
+    // Value of redshift table r at redshift z (rates_and_rate_tables.C.template:
+    // 384-401 and the pi/ph branches of :402-449): linear in ln(1 + z) on the
+    // table's bin, the bin clamped to [0, n_zbins - 1]; 0 ("no_photo") unless
+    // z_bounds[0] <= z < z_bounds[1].
+    static double z_rate(const Data *d, const double *r, double z)
+    {
+        if (!(z >= d->z_bounds[0] && z < d->z_bounds[1])) return 0.0;
+        const double lbz = log(d->z_bounds[0] + 1.0);
+        int zbin_id = (int)(d->id_zbin * (log(z + 1.0) - lbz));
+        if (zbin_id <= 0) zbin_id = 0;
+        else if (zbin_id >= d->n_zbins) zbin_id = d->n_zbins - 1;
+        const double z1 = (lbz + (zbin_id) * d->d_zbin);
+        const double z2 = (lbz + (zbin_id + 1) * d->d_zbin);
+        const double zdef = (log(z + 1.0) - z1) / (z2 - z1);
+        return r[zbin_id] + zdef * (r[zbin_id + 1] - r[zbin_id]);
     }
 
     static int ensure_device(Data *d, DevState *s)
@@ -514,7 +540,9 @@ struct Host {
             std::vector<double2> host((size_t)nb * NTAB);
             for (int t = 0; t < NTAB; ++t) {
                 const double *r = table(d, t);
-                for (int b = 0; b < nb; ++b) host[(size_t)b * NTAB + t] = make_double2(r[b], r[b + 1]);
+                for (int b = 0; b < nb; ++b)
+                    host[(size_t)b * NTAB + t] = Net::ztab(t) >= 0 ? make_double2(0.0, 0.0)
+                                                                   : make_double2(r[b], r[b + 1]);
             }
             if (!s->pairs) DENGO_HIP_OK(hipMalloc(&s->pairs, host.size() * sizeof(double2)));
             DENGO_HIP_OK(hipMemcpy(s->pairs, host.data(), host.size() * sizeof(double2),
@@ -557,6 +585,10 @@ struct Host {
         e.tb.T1 = d->bounds[1];
         e.zp1 = z + 1.0;
         e.comp4 = pow(e.zp1, 4);
+        e.z = z;
+        for (int k = 0; k < prim::kMaxZTables; ++k) e.zr[k] = 0.0;
+        for (int t = 0; t < NTAB; ++t)
+            if (Net::ztab(t) >= 0) e.zr[Net::ztab(t)] = z_rate(d, table(d, t), z);
         return e;
     }
 
@@ -580,11 +612,14 @@ struct Host {
             return set_error("%s holds %d tables x %d bins, the struct needs %d x %d", path, nt,
                              nT, NTAB, Tr::NT);
         }
+        std::vector<double> row((size_t)nT);
         for (int t = 0; t < nt; ++t) {
-            if (fread((double *)table(d, t), 8, nT, f) != (size_t)nT) {
+            // every row holds nT values; a redshift table keeps its first NZ
+            if (fread(row.data(), 8, nT, f) != (size_t)nT) {
                 fclose(f);
                 return set_error("%s: short read", path);
             }
+            memcpy((double *)table(d, t), row.data(), sizeof(double) * table_len(t));
         }
         fclose(f);
         set_grid(d, hdr[0], hdr[1]);
@@ -603,7 +638,7 @@ struct Host {
         for (int t = 0; t < NTAB; ++t) {
             char name[128];
             snprintf(name, sizeof(name), "/%s", Tr::table_names()[t] + 2);
-            if (dengo_h5_read(f, name, (double *)table(d, t), Tr::NT) != 0) {
+            if (dengo_h5_read(f, name, (double *)table(d, t), table_len(t)) != 0) {
                 char why[256];
                 snprintf(why, sizeof(why), "%s", last_error_text());
                 dengo_h5_close(f);
@@ -824,8 +859,9 @@ struct Host {
         if (ntables != NTAB || nT != Tr::NT)
             return set_error("expected %d tables x %d bins, got %d x %d", NTAB, Tr::NT, ntables, nT);
         if (!(T0 > 0) || !(T1 > T0)) return set_error("bad temperature bounds");
+        // rows of nT values; a redshift table keeps the first NZ of its row
         for (int t = 0; t < ntables; ++t)
-            memcpy((double *)table(d, t), tables + (size_t)t * nT, sizeof(double) * nT);
+            memcpy((double *)table(d, t), tables + (size_t)t * nT, sizeof(double) * table_len(t));
         set_grid(d, T0, T1);
         state(d)->stale = true;
         return 0;

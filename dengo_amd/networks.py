@@ -85,3 +85,47 @@ def oxygen_tables(seed=3, T_bounds=(1e4, 1e9), n_bins=1024):
         _TABLE_CACHE[key] = (net.table_names(), net.table_block())
     names, block = _TABLE_CACHE[key]
     return list(names), block.copy()
+
+
+PHOTO_REACTIONS = ("k01", "k02", "k03", "k04", "k05", "k06", "k07", "k08", "k09", "k10", "HI_pi")
+PHOTO_COOLING = ("brem", "ciHI", "compton", "gloverabel08", "HI_ph", "reHII")
+
+
+def _photo_shape(z):
+    """Smooth synthetic redshift dependence of the ultraviolet background:
+    rises as (1 + z)^1.5 to a broad peak near z = 2 and falls after it."""
+    z = np.asarray(z, dtype=np.float64)
+    return (1.0 + z) ** 1.5 * np.exp(-0.125 * (z - 2.0) ** 2)
+
+
+def _register_photo():
+    """H photoionisation HI + photon -> HII + de (reaction HI_pi) and its
+    photoheating (cooling action HI_ph, + HI_ph * HI), named as
+    dengo/reaction_classes.py:200-271 and :443-514 name them (<ion>_pi,
+    <ion>_ph): tables over the redshift grid, synthetic HM12-like shapes."""
+    from .reaction_classes import CoolingAction, Reaction, species_registry
+    HI, HII, de = (species_registry[n] for n in ("HI", "HII", "de"))
+    Reaction("HI_pi", lambda net: 4.0e-13 * _photo_shape(net.z), [(1, HI)],
+             [(1, HII), (1, de)])
+    ca = CoolingAction("HI_ph", "HI_ph * HI")
+    ca.tables["HI_ph"] = lambda net: 2.5e-24 * _photo_shape(net.z)
+
+
+def primordial_photo_network(T_bounds=(1.0, 1e9), n_bins=1024, z_bounds=(0.0, 10.0),
+                             n_z_bins=100, stop_time=1e12):
+    """The primordial species with a reduced reaction set (k01-k10) and
+    cooling set (brem, ciHI, compton, gloverabel08, reHII) plus H
+    photoionisation and photoheating on redshift tables (100 bins over
+    z = 0-10): a network the hand-written primordial device code cannot
+    serve, emitted by the generator."""
+    primordial_rates.setup_primordial()
+    _register_photo()
+    net = ChemicalNetwork(stop_time=stop_time)
+    for name in PHOTO_COOLING:
+        net.add_cooling(name)
+    for name in PHOTO_REACTIONS:
+        net.add_reaction(name)
+    net.init_temperature(T_bounds, n_bins)
+    net.init_redshift(z_bounds, n_z_bins)
+    assert tuple(net.species_list()) == PRIMORDIAL_SPECIES, net.species_list()
+    return net

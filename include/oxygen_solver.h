@@ -15,6 +15,10 @@
 #define OXYGEN_T_LO 10000.0
 #define OXYGEN_T_HI 1000000000.0
 #define OXYGEN_STOP_TIME 1000000000000.0
+/* Redshift grid of the pi/ph tables (rates_and_rate_tables.C.template:31-36). */
+#define OXYGEN_NZ 1
+#define OXYGEN_Z_LO 0.0
+#define OXYGEN_Z_HI 0.0
 
 typedef struct oxygen_data_ {
     double dbin;
@@ -144,6 +148,8 @@ typedef struct oxygen_data_ {
     X(c_O_7_c_O_7_c) \
     X(c_O_8_c_O_8_c) \
     X(c_O_9_c_O_9_c)
+/* Values per table in struct order: the T grid, or the z grid for a redshift table. */
+#define OXYGEN_TABLE_LENGTHS {1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024}
 
 #include <stddef.h>
 

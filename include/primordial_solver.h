@@ -15,6 +15,10 @@
 #define PRIMORDIAL_T_LO 1.0
 #define PRIMORDIAL_T_HI 1000000000.0
 #define PRIMORDIAL_STOP_TIME 1000000000000.0
+/* Redshift grid of the pi/ph tables (rates_and_rate_tables.C.template:31-36). */
+#define PRIMORDIAL_NZ 1
+#define PRIMORDIAL_Z_LO 0.0
+#define PRIMORDIAL_Z_HI 0.0
 
 typedef struct primordial_data_ {
     double dbin;
@@ -224,6 +228,8 @@ typedef struct primordial_data_ {
     X(c_reHeII2_reHeII2) \
     X(c_reHeIII_reHeIII) \
     X(c_reHII_reHII)
+/* Values per table in struct order: the T grid, or the z grid for a redshift table. */
+#define PRIMORDIAL_TABLE_LENGTHS {1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024}
 
 #include <stddef.h>
 

@@ -653,9 +653,93 @@ static int ma_jac(double *u, double *J, orc_prim_data *d)
     return 0;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Generated networks: the expressions come from dengo_amd.codegen, compiled */
+/* for the host; temperature, interpolation, redshift tables, the negative   */
+/* check and the energy row's division are the emitted solver's              */
+/* (rates_and_rate_tables.C.template:309-350, 362-449, 460-599).            */
+/* ------------------------------------------------------------------------ */
+double orc_z_rate(const double *r, int nz, double z0, double z1, double z)
+{
+    /* grid metadata as <n>_setup_data computes it (:31-36) */
+    const int n_zbins = nz - 1;
+    if (n_zbins <= 0 || !(z >= z0 && z < z1)) return 0.0;   /* no_photo */
+    const double d_zbin = (log(z1 + 1.0) - log(z0 + 1.0)) / n_zbins;
+    const double id_zbin = (double)(1.0L / d_zbin);
+    const double lbz = log(z0 + 1.0);
+    int zbin_id = (int)(id_zbin * (log(z + 1.0) - lbz));
+    if (zbin_id <= 0) zbin_id = 0;
+    else if (zbin_id >= n_zbins) zbin_id = n_zbins - 1;
+    const double za = (lbz + (zbin_id) * d_zbin);
+    const double zb = (lbz + (zbin_id + 1) * d_zbin);
+    const double zdef = (log(z + 1.0) - za) / (zb - za);
+    return r[zbin_id] + zdef * (r[zbin_id + 1] - r[zbin_id]);
+}
+
+static void gen_temperature(orc_prim_data *d, const double *u)
+{
+    const orc_gen_net *g = d->gen;
+    const int ge = g->n - 1;
+    const double kb = 1.3806504e-16, mh = 1.67e-24;
+    const double gamma = 5.e0 / 3.e0, gammaH2 = 7.e0 / 5.e0;
+    double density = 0.0, gsum = 0.0;
+    for (int s = 0; s < g->n; ++s)
+        if (s != ge && s != g->de) density += g->W[s] * u[s];
+    for (int s = 0; s < g->n; ++s)
+        if (s != ge) gsum += u[s] / ((g->h2gamma[s] ? gammaH2 : gamma) - 1.0);
+    double T = density * u[ge] * mh / (kb * gsum);
+    if (T < d->bounds[0]) T = d->bounds[0];
+    else if (T > d->bounds[1]) T = d->bounds[1];
+    d->T = T;
+    d->logT = (d->flags & ORC_FLAG_PORTABLE_LOG) ? orc_plog(T) : log(T);
+    d->invT = 1.0 / T;
+}
+
+static double gen_mdensity(const orc_gen_net *g, const double *u)
+{
+    double m = 0.0;
+    for (int s = 0; s < g->n; ++s)
+        if (s != g->n - 1 && s != g->de) m += u[s];
+    return m * 1.67e-24;
+}
+
+static int gen_rhs(double *u, double *f, orc_prim_data *d)
+{
+    const orc_gen_net *g = d->gen;
+    const int n = g->n, ge = n - 1;
+    gen_temperature(d, u);
+    orc_prim_interpolate(d);
+    for (int t = 0; t < g->ntab; ++t)
+        if (g->zslot[t] >= 0)
+            d->rs[t] = orc_z_rate(d->tables + (size_t)t * d->nT, g->nz, g->z_bounds[0],
+                                  g->z_bounds[1], d->current_z);
+    for (int s = 0; s < n; ++s)
+        if (u[s] < 0.0) return 1;
+    const double md = gen_mdensity(g, u);
+    g->rhs(u, d->rs, d->T, d->current_z, f);
+    f[ge] = f[ge] / md;
+    return 0;
+}
+
+static int gen_jac(double *u, double *J, orc_prim_data *d)
+{
+    const orc_gen_net *g = d->gen;
+    const int n = g->n, ge = n - 1;
+    for (int s = 0; s < n; ++s)
+        if (u[s] < 0.0) return 1;
+    const double md = gen_mdensity(g, u);
+    g->jac(u, d->rs, d->T, d->current_z, J);
+    for (int col = 0; col < n; ++col) J[col * n + ge] = J[col * n + ge] / md;
+    return 0;
+}
+
 static int any_rhs(double *u, double *f, int nstrip, int nchem, void *sdata)
 {
     orc_prim_data *d = (orc_prim_data *)sdata;
+    if (d->gen) {
+        d->rhs_calls++;
+        return gen_rhs(u, f, d);
+    }
     if (!d->net) return orc_prim_rhs(u, f, nstrip, nchem, sdata);
     d->rhs_calls++;
     return ma_rhs(u, f, d);
@@ -664,6 +748,7 @@ static int any_rhs(double *u, double *f, int nstrip, int nchem, void *sdata)
 static int any_jac(double *u, double *J, int nstrip, int nchem, void *sdata)
 {
     orc_prim_data *d = (orc_prim_data *)sdata;
+    if (d->gen) return gen_jac(u, J, d);
     if (!d->net) return orc_prim_jac(u, J, nstrip, nchem, sdata);
     return ma_jac(u, J, d);
 }
@@ -881,9 +966,9 @@ static double evolve_cell(orc_be_f be, orc_prim_data *d, double dtf, double dt, 
                           cell_stats *st)
 {
     const int N = d->n;
-    const int ge = N - 1, de = d->net ? d->net->de : DE;
-    const double *Wt = d->net ? d->net->W : W;
-    const double *Fe = d->net ? d->net->free_e : FREE_E;
+    const int ge = N - 1, de = d->gen ? d->gen->de : (d->net ? d->net->de : DE);
+    const double *Wt = d->gen ? d->gen->W : (d->net ? d->net->W : W);
+    const double *Fe = d->gen ? d->gen->free_e : (d->net ? d->net->free_e : FREE_E);
     double scale[ORC_MAXN], prev[ORC_MAXN], u0[ORC_MAXN], s[ORC_MAXN], gu[ORC_MAXN],
            Ju[ORC_MAXN * ORC_MAXN];
     if (!input_is_number) {
@@ -940,14 +1025,14 @@ static double evolve_cell(orc_be_f be, orc_prim_data *d, double dtf, double dt, 
     return ttot;
 }
 
-static double evolve_all(const orc_ma_net *net, double dtf, double *dt, double z, double *input,
-                         double *rtol, double *atol, long long dims, const double *tables,
-                         int nT, double T0, double T1, int input_is_number, int *status,
-                         int *nsub, int *nrej, int *nsweep, double *dt_out, double *T_out,
-                         orc_be_f be, int nthreads, int flags)
+static double evolve_all(const orc_ma_net *net, const orc_gen_net *gen, double dtf, double *dt,
+                         double z, double *input, double *rtol, double *atol, long long dims,
+                         const double *tables, int nT, double T0, double T1,
+                         int input_is_number, int *status, int *nsub, int *nrej, int *nsweep,
+                         double *dt_out, double *T_out, orc_be_f be, int nthreads, int flags)
 {
     const double dt_in = *dt;
-    const int N = net ? net->n : ORC_PRIM_N;
+    const int N = gen ? gen->n : (net ? net->n : ORC_PRIM_N);
     double tmin = 1e300, dtmin = dt_in;
     long long imin = -1;
     if (be == NULL) be = orc_BE_chem_solve;
@@ -964,6 +1049,11 @@ static double evolve_all(const orc_ma_net *net, double dtf, double *dt, double z
             d.net = net;
             d.n = net->n;
             d.ntab = net->ntab;
+        }
+        if (gen) {
+            d.gen = gen;
+            d.n = gen->n;
+            d.ntab = gen->ntab;
         }
         double t = evolve_cell(be, &d, dtf, dt_in, z, input + c * N, rtol + c * N,
                                atol + c * N, input_is_number, &st);
@@ -995,7 +1085,7 @@ double orc_ma_evolve(const orc_ma_net *net, double dtf, double *dt, double z, do
                      int *nsub, int *nrej, int *nsweep, double *dt_out, double *T_out,
                      orc_be_f be, int nthreads, int flags)
 {
-    return evolve_all(net, dtf, dt, z, input, rtol, atol, dims, tables, nT, T0, T1,
+    return evolve_all(net, NULL, dtf, dt, z, input, rtol, atol, dims, tables, nT, T0, T1,
                       input_is_number, status, nsub, nrej, nsweep, dt_out, T_out, be, nthreads,
                       flags);
 }
@@ -1031,7 +1121,7 @@ double orc_prim_evolve(double dtf, double *dt, double z, double *input,
                        int *nsweep, double *dt_out, double *T_out,
                        orc_be_f be, int nthreads, int flags)
 {
-    return evolve_all(NULL, dtf, dt, z, input, rtol, atol, dims, tables, nT, T0, T1,
+    return evolve_all(NULL, NULL, dtf, dt, z, input, rtol, atol, dims, tables, nT, T0, T1,
                       input_is_number, status, nsub, nrej, nsweep, dt_out, T_out, be, nthreads,
                       flags);
 }
@@ -1238,5 +1328,41 @@ long long orc_big_eval(const orc_big_net *net, const double *Tcell, const double
     }
     free(k);
     free(uc);
+    return nneg;
+}
+
+double orc_gen_evolve(const orc_gen_net *net, double dtf, double *dt, double z, double *input,
+                      double *rtol, double *atol, long long dims, const double *tables,
+                      int nT, double T0, double T1, int input_is_number, int *status,
+                      int *nsub, int *nrej, int *nsweep, double *dt_out, double *T_out,
+                      int nthreads, int flags)
+{
+    return evolve_all(NULL, net, dtf, dt, z, input, rtol, atol, dims, tables, nT, T0, T1,
+                      input_is_number, status, nsub, nrej, nsweep, dt_out, T_out, NULL,
+                      nthreads, flags);
+}
+
+int orc_gen_eval(const orc_gen_net *net, const double *u, long long dims, double z,
+                 const double *tables, int nT, double T0, double T1, double *f, double *J,
+                 double *T, double *rates, int flags)
+{
+    const int n = net->n;
+    int nneg = 0;
+    for (long long c = 0; c < dims; ++c) {
+        orc_prim_data d;
+        double uc[ORC_MAXN];
+        orc_prim_setup(&d, tables, nT, T0, T1);
+        d.flags = flags;
+        d.gen = net;
+        d.n = n;
+        d.ntab = net->ntab;
+        d.current_z = z;
+        memcpy(uc, u + c * n, sizeof(double) * n);
+        int rv = any_rhs(uc, f + c * n, 1, n, &d);
+        if (rv) nneg++;
+        else if (J) any_jac(uc, J + c * n * n, 1, n, &d);
+        if (T) T[c] = d.T;
+        if (rates) memcpy(rates + c * net->ntab, d.rs, sizeof(double) * net->ntab);
+    }
     return nneg;
 }

@@ -63,6 +63,23 @@ typedef struct orc_ma_net {
     orc_cool cool[ORC_MAXTAB];
 } orc_ma_net;
 
+/* A network whose species totals and cooling sum come from functions: the
+ * generated expressions of dengo_amd.codegen compiled for the host.  The
+ * oracle supplies everything around them (temperature, interpolation,
+ * redshift tables, the negative-species check, the division of the energy
+ * row by the mass density, BE/Newton and the substep driver). */
+typedef void (*orc_gen_fn)(const double *u, const double *rs, double T, double z, double *out);
+typedef struct orc_gen_net {
+    int n, ntab, de;                   /* energy index is n - 1 */
+    double W[ORC_MAXN], free_e[ORC_MAXN];
+    int h2gamma[ORC_MAXN];
+    int zslot[ORC_MAXTAB];             /* >= 0: a redshift table */
+    double z_bounds[2];
+    int nz;                            /* points of the redshift grid */
+    orc_gen_fn rhs;                    /* f[s] = species total, f[ge] = cooling sum */
+    orc_gen_fn jac;                    /* J[col * n + row] of the same forms */
+} orc_gen_net;
+
 /* Per-call state handed to the RHS/Jacobian callbacks (nstrip must be 1). */
 typedef struct orc_prim_data {
     double bounds[2];
@@ -79,6 +96,7 @@ typedef struct orc_prim_data {
     int flags;              /* ORC_FLAG_* */
     const orc_ma_net *net;  /* NULL: the primordial network */
     int n, ntab;
+    const orc_gen_net *gen; /* non-NULL: a generated network */
 } orc_prim_data;
 
 /* Accepted-substep cap (default 50000, as the reference). */
@@ -145,6 +163,22 @@ long long orc_big_eval(const orc_big_net *net, const double *Tcell, const double
  * the primordial network): T clamped below at 1 K, no upper clamp. */
 void orc_temperature_from_mass_density(const orc_ma_net *net, const double *input, int nstrip,
                                        int nchem, double *T);
+
+/* Value at redshift z of a table r on nz points log-spaced in 1 + z over
+ * [z0, z1] (rates_and_rate_tables.C.template:384-401 and the pi/ph branches
+ * of :402-449): linear in ln(1 + z), 0 outside z0 <= z < z1. */
+double orc_z_rate(const double *r, int nz, double z0, double z1, double z);
+
+/* Generated-network counterparts of orc_prim_evolve / orc_prim_eval; tables
+ * [ntab][nT], a redshift table's first nz values in its row. */
+double orc_gen_evolve(const orc_gen_net *net, double dtf, double *dt, double z, double *input,
+                      double *rtol, double *atol, long long dims, const double *tables,
+                      int nT, double T0, double T1, int input_is_number, int *status,
+                      int *nsub, int *nrej, int *nsweep, double *dt_out, double *T_out,
+                      int nthreads, int flags);
+int orc_gen_eval(const orc_gen_net *net, const double *u, long long dims, double z,
+                 const double *tables, int nT, double T0, double T1, double *f, double *J,
+                 double *T, double *rates, int flags);
 
 /* Mass-action network counterparts of orc_prim_evolve / orc_prim_eval. */
 /* One BE_chem_solve (be; NULL selects orc_BE_chem_solve) of a strip of

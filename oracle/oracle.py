@@ -357,6 +357,159 @@ def temperature_from_mass_density(inp, net=None):
     return T
 
 
+# ---------------------------------------------------------------------------
+# Generated networks (dengo_amd.codegen, kind "gen"): the network's printed
+# expressions compiled for the host, everything around them restated here.
+# ---------------------------------------------------------------------------
+GEN_DIR = os.path.join(HERE, "_gen")
+
+
+class GenNet(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("ntab", ctypes.c_int), ("de", ctypes.c_int),
+                ("W", ctypes.c_double * 16), ("free_e", ctypes.c_double * 16),
+                ("h2gamma", ctypes.c_int * 16), ("zslot", ctypes.c_int * 64),
+                ("z_bounds", ctypes.c_double * 2), ("nz", ctypes.c_int),
+                ("rhs", ctypes.c_void_p), ("jac", ctypes.c_void_p)]
+
+
+_GEN_WRAPPER = """// Host build of the generated expressions of one network (test infrastructure).
+#define DENGO_GEN_FN inline
+#include "net_%(n)s.h"
+#include "%(n)s_solver.h"
+using G = dengo::net_%(n)s::Gen;
+extern "C" {
+void gen_rhs(const double *u, const double *rs, double T, double z, double *f)
+{
+    G::rhs(*reinterpret_cast<const double (*)[G::N]>(u),
+           *reinterpret_cast<const double (*)[G::NTAB]>(rs), T, z,
+           *reinterpret_cast<double (*)[G::N]>(f));
+}
+void gen_jac(const double *u, const double *rs, double T, double z, double *J)
+{
+    G::jac(*reinterpret_cast<const double (*)[G::N]>(u),
+           *reinterpret_cast<const double (*)[G::NTAB]>(rs), T, z,
+           *reinterpret_cast<double (*)[G::N * G::N]>(J));
+}
+void gen_layout(int *n, int *ntab, int *de, double *W, double *fe, int *h2, int *zslot,
+                double *zb, int *nz)
+{
+    *n = G::N;
+    *ntab = G::NTAB;
+    *de = G::DE;
+    for (int k = 0; k < G::N; ++k) {
+        W[k] = G::W[k];
+        fe[k] = G::FREE_E[k];
+        h2[k] = G::H2G[k] ? 1 : 0;
+    }
+    for (int t = 0; t < G::NTAB; ++t) zslot[t] = G::ZTAB[t];
+    zb[0] = %(N)s_Z_LO;
+    zb[1] = %(N)s_Z_HI;
+    *nz = %(N)s_NZ;
+}
+}
+"""
+
+
+def build_gen(name, solver_dir, include_dir):
+    """Compiles the generated network <name> (net_<name>.h and <name>_solver.h
+    in solver_dir) for the host into _gen/lib<name>_expr.so, without FP
+    contraction so each expression rounds as written."""
+    import subprocess
+    os.makedirs(GEN_DIR, exist_ok=True)
+    src = os.path.join(GEN_DIR, "%s_expr.cc" % name)
+    with open(src, "w") as f:
+        f.write(_GEN_WRAPPER % {"n": name, "N": name.upper()})
+    lib = os.path.join(GEN_DIR, "lib%s_expr.so" % name)
+    subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC",
+                    "-shared", "-I", solver_dir, "-I", include_dir, "-o", lib, src], check=True)
+    return lib
+
+
+_GEN = {}
+
+
+def gen_net(name):
+    """The orc_gen_net of a generated network built by build_gen (kept alive
+    with its library)."""
+    if name not in _GEN:
+        path = os.path.join(GEN_DIR, "lib%s_expr.so" % name)
+        if not os.path.exists(path):
+            raise RuntimeError("%s not built: __graft_entry__.build() builds it" % path)
+        lib = ctypes.CDLL(path)
+        net = GenNet()
+        n, ntab, de, nz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        lib.gen_layout(ctypes.byref(n), ctypes.byref(ntab), ctypes.byref(de), net.W,
+                       net.free_e, net.h2gamma, net.zslot, net.z_bounds, ctypes.byref(nz))
+        net.n, net.ntab, net.de, net.nz = n.value, ntab.value, de.value, nz.value
+        net.rhs = ctypes.cast(lib.gen_rhs, ctypes.c_void_p).value
+        net.jac = ctypes.cast(lib.gen_jac, ctypes.c_void_p).value
+        net._lib = lib
+        _GEN[name] = net
+    return _GEN[name]
+
+
+def _gen_lib():
+    lib = load()
+    if not getattr(lib, "_gen_ready", False):
+        lib.orc_gen_evolve.restype = ctypes.c_double
+        lib.orc_gen_evolve.argtypes = [ctypes.POINTER(GenNet), ctypes.c_double, _dp,
+                                       ctypes.c_double, _dp, _dp, _dp, ctypes.c_longlong, _dp,
+                                       ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_int, _ip, _ip, _ip, _ip, _dp, _dp, ctypes.c_int,
+                                       ctypes.c_int]
+        lib.orc_gen_eval.restype = ctypes.c_int
+        lib.orc_gen_eval.argtypes = [ctypes.POINTER(GenNet), _dp, ctypes.c_longlong,
+                                     ctypes.c_double, _dp, ctypes.c_int, ctypes.c_double,
+                                     ctypes.c_double, _dp, _dp, _dp, _dp, ctypes.c_int]
+        lib.orc_z_rate.restype = ctypes.c_double
+        lib.orc_z_rate.argtypes = [_dp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_double]
+        lib._gen_ready = True
+    return lib
+
+
+def z_rate(r, z_bounds, z):
+    """The oracle's value of redshift table r (values on the z grid) at z."""
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    return _gen_lib().orc_z_rate(_ptr(r), r.size, z_bounds[0], z_bounds[1], z)
+
+
+def gen_eval(net, u, tables, T_bounds, z=-1.0, flags=0):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    n, N = u.shape
+    f = np.zeros((n, N))
+    J = np.zeros((n, N, N))
+    T = np.zeros(n)
+    rates = np.zeros((n, net.ntab))
+    tab = np.ascontiguousarray(tables, dtype=np.float64)
+    nneg = _gen_lib().orc_gen_eval(ctypes.byref(net), _ptr(u), n, z, _ptr(tab), tab.shape[1],
+                                   T_bounds[0], T_bounds[1], _ptr(f), _ptr(J), _ptr(T),
+                                   _ptr(rates), flags)
+    return f, J, T, rates, nneg
+
+
+def gen_evolve(net, inp, rtol, atol, tables, dtf, T_bounds, dt=-1.0, z=-1.0,
+               input_is_number=False, nthreads=0, flags=0, max_substeps=50000):
+    lib = _gen_lib()
+    lib.orc_set_max_substeps(max_substeps)
+    out = np.array(inp, dtype=np.float64, copy=True, order="C")
+    atol_o = np.array(atol, dtype=np.float64, copy=True, order="C")
+    rtol_c = np.array(rtol, dtype=np.float64, copy=True, order="C")
+    n = out.shape[0]
+    status, nsub, nrej, nsweep = (np.zeros(n, dtype=np.int32) for _ in range(4))
+    dt_out, T_out = np.zeros(n), np.zeros(n)
+    tab = np.ascontiguousarray(tables, dtype=np.float64)
+    dtc = ctypes.c_double(dt)
+    ttot = lib.orc_gen_evolve(
+        ctypes.byref(net), dtf, ctypes.byref(dtc), z, _ptr(out), _ptr(rtol_c), _ptr(atol_o), n,
+        _ptr(tab), tab.shape[1], T_bounds[0], T_bounds[1], int(input_is_number),
+        _ptr(status, _ip), _ptr(nsub, _ip), _ptr(nrej, _ip), _ptr(nsweep, _ip), _ptr(dt_out),
+        _ptr(T_out), nthreads, flags)
+    lib.orc_set_max_substeps(50000)
+    return dict(state=out, atol=atol_o, ttot=ttot, dt=dtc.value, status=status, nsub=nsub,
+                nrej=nrej, nsweep=nsweep, dt_cell=dt_out, T=T_out)
+
+
 class StripSolver(object):
     """The oracle with the interface of dengo_amd's NetworkSolver strip
     methods (be_chem_solve, ensure_electron_consistency, set_current_z,

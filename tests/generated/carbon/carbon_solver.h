@@ -15,6 +15,10 @@
 #define CARBON_T_LO 10000.0
 #define CARBON_T_HI 1000000000.0
 #define CARBON_STOP_TIME 1000000000000.0
+/* Redshift grid of the pi/ph tables (rates_and_rate_tables.C.template:31-36). */
+#define CARBON_NZ 1
+#define CARBON_Z_LO 0.0
+#define CARBON_Z_HI 0.0
 
 typedef struct carbon_data_ {
     double dbin;
@@ -120,6 +124,8 @@ typedef struct carbon_data_ {
     X(c_C_5_c_C_5_c) \
     X(c_C_6_c_C_6_c) \
     X(c_C_7_c_C_7_c)
+/* Values per table in struct order: the T grid, or the z grid for a redshift table. */
+#define CARBON_TABLE_LENGTHS {1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024}
 
 #include <stddef.h>
 

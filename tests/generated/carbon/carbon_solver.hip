@@ -15,6 +15,14 @@ struct Traits {
     static constexpr double T_LO = CARBON_T_LO;
     static constexpr double T_HI = CARBON_T_HI;
     static constexpr double STOP_TIME = CARBON_STOP_TIME;
+    static constexpr int NZ = CARBON_NZ;
+    static constexpr double Z_LO = CARBON_Z_LO;
+    static constexpr double Z_HI = CARBON_Z_HI;
+    static const int *table_lengths()
+    {
+        static const int l[] = CARBON_TABLE_LENGTHS;
+        return l;
+    }
     static const size_t *table_offsets()
     {
         static const size_t o[] = {

@@ -1,0 +1,234 @@
+/* Drop-in header for the primphoto network (generated by dengo_amd.codegen).
+   Struct fields and order follow dengo/templates/rates_and_rate_tables.h.template:30-83;
+   entry points follow :85-108 with C linkage. */
+#ifndef PRIMPHOTO_SOLVER_H
+#define PRIMPHOTO_SOLVER_H
+
+#ifndef MAX_NCELLS
+#define MAX_NCELLS 1024
+#endif
+#define PRIMPHOTO_NSPECIES 10
+#define PRIMPHOTO_NTABLES 23
+#define PRIMPHOTO_NT 1024
+/* Temperature grid of the tables and <name>_main's dtf
+   (rates_and_rate_tables.C.template:25-26, 110). */
+#define PRIMPHOTO_T_LO 1.0
+#define PRIMPHOTO_T_HI 1000000000.0
+#define PRIMPHOTO_STOP_TIME 1000000000000.0
+/* Redshift grid of the pi/ph tables (rates_and_rate_tables.C.template:31-36). */
+#define PRIMPHOTO_NZ 100
+#define PRIMPHOTO_Z_LO 0.0
+#define PRIMPHOTO_Z_HI 10.0
+
+typedef struct primphoto_data_ {
+    double dbin;
+    double idbin;
+    double bounds[2];
+    int nbins;
+    double d_zbin;
+    double id_zbin;
+    double z_bounds[2];
+    int n_zbins;
+    double current_z;
+    double zdef;
+    double dz;
+    double Ts[MAX_NCELLS];
+    double Tdef[MAX_NCELLS];
+    double dT[MAX_NCELLS];
+    double logTs[MAX_NCELLS];
+    double invTs[MAX_NCELLS];
+    double dTs_ge[MAX_NCELLS];
+    double r_HI_pi[100];
+    double rs_HI_pi[MAX_NCELLS];
+    double drs_HI_pi[MAX_NCELLS];
+    double r_k01[1024];
+    double rs_k01[MAX_NCELLS];
+    double drs_k01[MAX_NCELLS];
+    double r_k02[1024];
+    double rs_k02[MAX_NCELLS];
+    double drs_k02[MAX_NCELLS];
+    double r_k03[1024];
+    double rs_k03[MAX_NCELLS];
+    double drs_k03[MAX_NCELLS];
+    double r_k04[1024];
+    double rs_k04[MAX_NCELLS];
+    double drs_k04[MAX_NCELLS];
+    double r_k05[1024];
+    double rs_k05[MAX_NCELLS];
+    double drs_k05[MAX_NCELLS];
+    double r_k06[1024];
+    double rs_k06[MAX_NCELLS];
+    double drs_k06[MAX_NCELLS];
+    double r_k07[1024];
+    double rs_k07[MAX_NCELLS];
+    double drs_k07[MAX_NCELLS];
+    double r_k08[1024];
+    double rs_k08[MAX_NCELLS];
+    double drs_k08[MAX_NCELLS];
+    double r_k09[1024];
+    double rs_k09[MAX_NCELLS];
+    double drs_k09[MAX_NCELLS];
+    double r_k10[1024];
+    double rs_k10[MAX_NCELLS];
+    double drs_k10[MAX_NCELLS];
+    double c_brem_brem[1024];
+    double cs_brem_brem[MAX_NCELLS];
+    double dcs_brem_brem[MAX_NCELLS];
+    double c_ciHI_ciHI[1024];
+    double cs_ciHI_ciHI[MAX_NCELLS];
+    double dcs_ciHI_ciHI[MAX_NCELLS];
+    double c_compton_comp[1024];
+    double cs_compton_comp[MAX_NCELLS];
+    double dcs_compton_comp[MAX_NCELLS];
+    double c_gloverabel08_gael[1024];
+    double cs_gloverabel08_gael[MAX_NCELLS];
+    double dcs_gloverabel08_gael[MAX_NCELLS];
+    double c_gloverabel08_gaH2[1024];
+    double cs_gloverabel08_gaH2[MAX_NCELLS];
+    double dcs_gloverabel08_gaH2[MAX_NCELLS];
+    double c_gloverabel08_gaHe[1024];
+    double cs_gloverabel08_gaHe[MAX_NCELLS];
+    double dcs_gloverabel08_gaHe[MAX_NCELLS];
+    double c_gloverabel08_gaHI[1024];
+    double cs_gloverabel08_gaHI[MAX_NCELLS];
+    double dcs_gloverabel08_gaHI[MAX_NCELLS];
+    double c_gloverabel08_gaHp[1024];
+    double cs_gloverabel08_gaHp[MAX_NCELLS];
+    double dcs_gloverabel08_gaHp[MAX_NCELLS];
+    double c_gloverabel08_gphdl[1024];
+    double cs_gloverabel08_gphdl[MAX_NCELLS];
+    double dcs_gloverabel08_gphdl[MAX_NCELLS];
+    double c_gloverabel08_gpldl[1024];
+    double cs_gloverabel08_gpldl[MAX_NCELLS];
+    double dcs_gloverabel08_gpldl[MAX_NCELLS];
+    double c_HI_ph_HI_ph[100];
+    double cs_HI_ph_HI_ph[MAX_NCELLS];
+    double dcs_HI_ph_HI_ph[MAX_NCELLS];
+    double c_reHII_reHII[1024];
+    double cs_reHII_reHII[MAX_NCELLS];
+    double dcs_reHII_reHII[MAX_NCELLS];
+    int bin_id[MAX_NCELLS];
+    int ncells;
+    /* Device-resident tables and scratch, owned by the HIP library. */
+    void *device_state;
+} primphoto_data;
+
+/* Table fields in struct order, for code that walks every table. */
+/* Field names in solver order, as <name>_setup_data reports them. */
+#define PRIMPHOTO_FIELD_NAMES {"H2I", "H2II", "HI", "HII", "HM", "HeI", "HeII", "HeIII", "de", "ge"}
+#define PRIMPHOTO_TABLE_FIELDS(X) \
+    X(r_HI_pi) \
+    X(r_k01) \
+    X(r_k02) \
+    X(r_k03) \
+    X(r_k04) \
+    X(r_k05) \
+    X(r_k06) \
+    X(r_k07) \
+    X(r_k08) \
+    X(r_k09) \
+    X(r_k10) \
+    X(c_brem_brem) \
+    X(c_ciHI_ciHI) \
+    X(c_compton_comp) \
+    X(c_gloverabel08_gael) \
+    X(c_gloverabel08_gaH2) \
+    X(c_gloverabel08_gaHe) \
+    X(c_gloverabel08_gaHI) \
+    X(c_gloverabel08_gaHp) \
+    X(c_gloverabel08_gphdl) \
+    X(c_gloverabel08_gpldl) \
+    X(c_HI_ph_HI_ph) \
+    X(c_reHII_reHII)
+/* Values per table in struct order: the T grid, or the z grid for a redshift table. */
+#define PRIMPHOTO_TABLE_LENGTHS {100, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 100, 1024}
+
+#include <stddef.h>
+
+#include "dengo_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* Drop-in entry points (rates_and_rate_tables.C.template:17-54, 149-274,
+   278-304, 309-350, 460-599; the reference's double &dt is double *dt). */
+primphoto_data *primphoto_setup_data(int *NumberOfFields, char ***FieldNames);
+void primphoto_read_rate_tables(primphoto_data *data);
+void primphoto_read_cooling_tables(primphoto_data *data);
+void primphoto_free_data(primphoto_data *data);
+/* rates_and_rate_tables.C.template:56-147: reads primphoto_initial_conditions.h5,
+   integrates to PRIMPHOTO_STOP_TIME, writes primphoto_solution.h5 (the .bin forms of
+   dengo_amd.tables without HDF5); 0 on success, 1 on failure. */
+int primphoto_main(int argc, char **argv);
+double dengo_evolve_primphoto(double dtf, double *dt, double z, double *input,
+        double *rtol, double *atol, long long dims, primphoto_data *data);
+int calculate_rhs_primphoto(double *input, double *rhs, int nstrip, int nchem, void *sdata);
+int calculate_jacobian_primphoto(double *input, double *Joutput, int nstrip, int nchem, void *sdata);
+void primphoto_calculate_temperature(primphoto_data *data, double *input, int nstrip, int nchem);
+
+/* Backend extensions (no counterpart in the reference).
+   Allocate an empty struct; set tables from memory (tables[t * nT + i] in
+   struct order, T grid log-spaced on [T0, T1]); read the table blob from an
+   explicit path. */
+primphoto_data *primphoto_alloc_data(void);
+int primphoto_set_tables(primphoto_data *data, const double *tables, int ntables, int nT,
+        double T0, double T1);
+primphoto_data *primphoto_setup_data_from(const char *path, int *NumberOfFields, char ***FieldNames);
+/* sizeof(primphoto_data), for bindings that hold the struct as bytes. */
+size_t primphoto_data_bytes(void);
+/* Integrate dims cells held in device memory (AoS [dims][NSPECIES], updated
+   in place), enqueued on stream (a hipStream_t, NULL = default).  dt0 < 0
+   selects dtf/1e5.  With ttot_min non-NULL the call synchronises and returns
+   the smallest elapsed time and that cell's final dt. */
+int dengo_primphoto_evolve_device(primphoto_data *data, const dengo_options *opt, double dtf,
+        double dt0, double z, double *d_input, const double *d_rtol, double *d_atol,
+        long long dims, const dengo_cell_outputs *outputs, double *ttot_min,
+        double *dt_min, void *stream);
+/* Host-array form of dengo_evolve_primphoto with options and per-cell outputs. */
+double dengo_primphoto_evolve_host(primphoto_data *data, const dengo_options *opt, double dtf,
+        double *dt, double z, double *input, double *rtol, double *atol, long long dims,
+        int *status, int *nsub, int *nrej, int *nsweep, double *dt_cell, double *T_cell);
+/* Batched RHS f[dims][N], Jacobian J[dims][N][N] (column-major per cell, may
+   be NULL), temperature T[dims] and interpolated tables rates[dims][NTABLES]
+   (may be NULL) for number-density states on host memory; returns the number
+   of cells whose RHS saw a negative value, or a negative error code. */
+long long dengo_primphoto_eval(primphoto_data *data, const double *u, long long dims, double z,
+        double *f, double *J, double *T, double *rates);
+/* BE_chem_solve(calculate_rhs_primphoto, calculate_jacobian_primphoto, ...) as the Cython
+   harness calls it (dengo/solvers/BE_chem_solve.C:79-241,
+   rates_and_rate_tables_run.pyx.template:142) on host arrays: one backward-Euler
+   step of nstrip cells that share dt and the Newton convergence decision; u and
+   atol are scaled and unscaled in place as the reference does; the redshift is
+   data->current_z.  Returns 0 (converged), 1 (not converged in 10 sweeps, or a
+   failed RHS/Jacobian/NaN update), or a negative error code. */
+int dengo_primphoto_be_chem_solve(primphoto_data *data, double *u, double dt, double *rtol,
+        double *atol, int nstrip, int nchem, double *scaling);
+/* ensure_electron_consistency (rates_and_rate_tables.C.template:604-630): de of
+   cell i becomes the free-electron total of cells 0..i (the reference's total
+   runs across the strip). */
+void ensure_electron_consistency_primphoto(double *input, int nstrip, int nchem);
+/* temperature_from_mass_density (rates_and_rate_tables.C.template:635-668): the
+   temperature of nstrip cells given as mass densities (ge in erg/g), clamped below
+   at 1 K only; computed on the GPU. */
+void temperature_from_mass_density_primphoto(double *input, int nstrip, int nchem,
+        double *strip_temperature);
+/* The same two helpers under the reference's unsuffixed names
+   (rates_and_rate_tables.h.template:106-107): this library holds one network. */
+void ensure_electron_consistency(double *input, int nstrip, int nchem);
+void temperature_from_mass_density(double *input, int nstrip, int nchem,
+        double *strip_temperature);
+/* Sets data->current_z, the redshift the strip solve uses. */
+void dengo_primphoto_set_current_z(primphoto_data *data, double z);
+/* Copies data->Ts (temperatures of the last strip RHS evaluation) for
+   min(nstrip, MAX_NCELLS) cells; returns that count. */
+int dengo_primphoto_strip_temperatures(const primphoto_data *data, double *T, int nstrip);
+#ifdef __cplusplus
+}
+static inline double dengo_evolve_primphoto(double dtf, double &dt, double z, double *input,
+        double *rtol, double *atol, long long dims, primphoto_data *data)
+{
+    return dengo_evolve_primphoto(dtf, &dt, z, input, rtol, atol, dims, data);
+}
+#endif
+
+#endif

@@ -24,7 +24,8 @@ def test_builtin_sources_are_generated():
     with np.errstate(all="ignore"):
         prim = primordial_network()
     oxy = oxygen_network()
-    assert codegen.render_instantiation(prim, "primordial") == _read(CSRC, "dengo_primordial.hip")
+    assert codegen.render_instantiation(prim, "primordial", kind="prim") == \
+        _read(CSRC, "dengo_primordial.hip")
     assert codegen.render_instantiation(oxy, "oxygen") == _read(CSRC, "dengo_oxygen.hip")
     assert codegen.render_device_network(oxy, "oxygen") == _read(CSRC, "net_oxygen.h")
     assert codegen.render_data_header(oxy, "oxygen") == _read(ROOT, "include", "oxygen_solver.h")

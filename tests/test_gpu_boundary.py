@@ -71,8 +71,7 @@ def test_unsuffixed_helpers_of_a_single_network_library():
     lib.ensure_electron_consistency_carbon(b.ctypes.data_as(_dp), b.shape[0], b.shape[1])
     assert np.array_equal(a, b)
     c = x.copy()
-    oracle.load()
-    oracle._strip_lib().orc_strip_electrons(
-        ctypes.cast(ctypes.byref(oracle.ion_chain_net(6, 12.0107)), ctypes.c_void_p),
-        c.ctypes.data_as(_dp), c.shape[0])
+    net = oracle.ion_chain_net(6, 12.0107)    # held: a byref of a temporary would dangle
+    oracle._strip_lib().orc_strip_electrons(ctypes.addressof(net), c.ctypes.data_as(_dp),
+                                            c.shape[0])
     assert np.array_equal(a, c)
