@@ -31,18 +31,10 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# Algorithmic FP64 operations per Newton sweep of the primordial network
-# (divisions and compares count as one): RHS 465 (T 36, ln T 29,
-# interpolation 144, mdensity 9, fluxes 46, accumulation 132, cooling 69),
-# residual/scaling 80, Jacobian 447, Jacobian scaling 310, Gauss_Elim 805
-# (45 multipliers, 570 update, 90 forward, 100 back-substitution), update and
-# convergence test 30.
-F_SWEEP = 2137
-# Algorithmic FP64 operations per Newton sweep of the dense 128-species solve
-# (the reference's algorithm, whatever elimination the kernel uses):
-# Gauss_Elim sum_{m=1}^{127} (2 m^2 + 3 m) + 128^2 = 1,422,528, Jacobian
-# assembly and scaling about 80,000, RHS about 10,000.
-F_SWEEP_DENSE = 1512528
+# Algorithmic FP64 operations per Newton sweep come from dengo_amd.flops:
+# counted on the network's symbolic forms as the reference's emitted solver
+# evaluates them, plus closed forms for the scaling, Gauss_Elim and update
+# (2,333 for the primordial network, 1,497,344 for the dense 128-species one).
 PEAK_FP64 = 78.6e12      # MI355X FP64 (vector and matrix) TFLOP/s
 # HBM bytes per launch of the FP64-fast persistent kernel on the 128^3
 # primordial workload: FETCH_SIZE + WRITE_SIZE from two rocprofv3 --pmc runs
@@ -136,6 +128,33 @@ def serving_leg(args, prim):
             a.free()
     oxy.close()
     syn.close()
+    return out
+
+
+def error_stats(x, st, x_ref, st_ref, atol, rtol, ns):
+    """Error measures of a run against a reference run over the cells both
+    take to dtf (numpy arrays): the largest relative error per cell over the
+    energy and the species above 1e-10 of the cell's species mass (the
+    relative error of species at the 1e-25 floor is ill-posed), and the
+    largest error in the solver's tolerance units |dx| / (atol + rtol |x|)."""
+    done = (st == 0) & (st_ref == 0)
+    out = {"status_agreement": float(np.mean(st == st_ref)), "cells_compared": int(done.sum())}
+    if not done.any():
+        return out
+    x, r, a, rt = x[done], x_ref[done], atol[done], rtol[done]
+    mass = np.abs(r[:, :ns]).sum(axis=1, keepdims=True)
+    major = np.concatenate([np.abs(r[:, :ns]) > 1e-10 * mass,
+                            np.ones((r.shape[0], r.shape[1] - ns), bool)], axis=1)
+    rel = np.abs(x - r) / np.maximum(np.abs(r), 1e-300)
+    mrel = np.where(major, rel, 0.0).max(axis=1)
+    tol = (np.abs(x - r) / (a + rt * np.abs(r))).max(axis=1)
+    out.update({
+        "cells_within_1e-6_major": float(np.mean(mrel <= 1e-6)),
+        "major_rel_p99_9": float(np.quantile(mrel, 0.999)),
+        "major_rel_max": float(mrel.max()),
+        "tol_units_p99_9": float(np.quantile(tol, 0.999)),
+        "tol_units_max": float(tol.max()),
+    })
     return out
 
 
@@ -241,7 +260,7 @@ def main():
     ap.add_argument("--no-cost-ordered", action="store_true",
                     help="skip the ordered reruns (cost and density) of the oxygen and dense legs")
     ap.add_argument("--cpu-cells", type=int, default=4096)
-    ap.add_argument("--cpu-repeats", type=int, default=3,
+    ap.add_argument("--cpu-repeats", type=int, default=5,
                     help="CPU baseline runs (the median is reported)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--oxygen-cells", type=int, default=1 << 20,
@@ -271,12 +290,16 @@ def main():
         sys.exit(2)
 
     import torch
+    from dengo_amd import flops as flopcount
     from dengo_amd import runtime
     from dengo_amd.ics import primordial_ics, tolerances
-    from dengo_amd.networks import primordial_tables
+    from dengo_amd.networks import primordial_network, primordial_tables
 
     dev = torch.device("cuda:0")
     names, tab = primordial_tables()
+    with np.errstate(all="ignore"):
+        sweep_ops = flopcount.sweep_flops(primordial_network())
+    F_SWEEP = sweep_ops["total"]
     solver = runtime.PrimordialSolver(tab)
     n = args.cells
     log("generating %d cells" % n)
@@ -352,31 +375,46 @@ def main():
     # CPU oracle): status agreement and per-cell max relative error over the
     # cells both runs integrate to dtf, beside the reference's own sensitivity
     # (the faithful kernel rerun with ge perturbed by one ulp)
+    x64_host = x64.cpu().numpy()
+    heads = {"fp64_persistent": x64_host[:min(args.cpu_cells, n)].copy()}
+
     def compare(x, st):
         done = torch.from_numpy((status == 0) & (st == 0)).to(dev)
         rel = ((x - x64).abs() / x64.abs().clamp_min(1e-300))[done].max(dim=1).values
         q = lambda p: float(torch.quantile(rel.float().cpu(), p)) if rel.numel() else 0.0
-        return {"bitwise_equal": bool(torch.equal(x, x64)),
-                "status_agreement": float((st == status).mean()),
-                "cells_within_1e-6": float((rel <= 1e-6).float().mean()) if rel.numel() else 1.0,
-                "p99_9_cell_max_rel": q(0.999), "max_rel": float(rel.max()) if rel.numel() else 0.0}
+        rec = {"bitwise_equal": bool(torch.equal(x, x64)),
+               "status_agreement": float((st == status).mean()),
+               "cells_within_1e-6": float((rel <= 1e-6).float().mean()) if rel.numel() else 1.0,
+               "p99_9_cell_max_rel": q(0.999), "max_rel": float(rel.max()) if rel.numel() else 0.0}
+        xh_ = x.cpu().numpy()
+        well = error_stats(xh_, st, x64_host, status, atol, rtol, 9)
+        rec.update({k: v for k, v in well.items() if k != "status_agreement"})
+        return rec, xh_[:min(args.cpu_cells, n)].copy(), st[:min(args.cpu_cells, n)].copy()
 
     st_out = torch.zeros(n, dtype=torch.int32, device=dev)
     xh, ah = d_inp.clone(), d_atol.clone()
     run(args.variant, xh, ah, {"status": st_out})
     parity = {"vs": "fp64_persistent, bit-identical to the CPU oracle",
-              args.variant: compare(xh, st_out.cpu().numpy())}
+              "metrics": "status_agreement over all cells; the rest over cells both runs take "
+                         "to dtf: cells_within_1e-6 and max_rel over every component, "
+                         "*_major over the energy and the species above 1e-10 of the cell's "
+                         "species mass, tol_units = |dx| / (atol + rtol |x|)"}
+    heads["fp64_persistent_status"] = status[:min(args.cpu_cells, n)].copy()
+    parity[args.variant], heads[args.variant], heads[args.variant + "_status"] = compare(
+        xh, st_out.cpu().numpy())
     xp = d_inp.clone()
     xp[:, 9] = torch.nextafter(xp[:, 9], torch.full_like(xp[:, 9], float("inf")))
     ap_ = d_atol.clone()
     run("fp64_persistent", xp, ap_, {"status": st_out})
-    parity["reference_noise_floor_ge_plus_1ulp"] = compare(xp, st_out.cpu().numpy())
+    fl = "reference_noise_floor_ge_plus_1ulp"
+    parity[fl], heads[fl], heads[fl + "_status"] = compare(xp, st_out.cpu().numpy())
     extra = {}
     for v in [v for v in args.also.split(",") if v and v != args.variant]:
         xe, ae = d_inp.clone(), d_atol.clone()
         _, ms_e = run(v, xe, ae, {"status": st_out})
         extra[v] = {"kernel_ms": ms_e, "cell_solves_per_s": n / (ms_e * 1e-3)}
-        extra[v].update(compare(xe, st_out.cpu().numpy()))
+        rec, heads[v], heads[v + "_status"] = compare(xe, st_out.cpu().numpy())
+        extra[v].update(rec)
         log("%s: %.1f ms kernel" % (v, ms_e))
     del xs, ats
 
@@ -386,13 +424,16 @@ def main():
     else:
         peak = PEAK_FP64
     achieved = flops / (kernel_ms * 1e-3)
-    roof = {"bound": "mfma", "compute_unit": "FP64 vector ALU (MI355X FP64 vector and "
-            "matrix peaks are both 78.6 TFLOP/s)", "achieved": achieved / 1e12,
+    roof = {"bound": "fp64-valu", "compute_unit": "FP64 vector ALU (MI355X FP64 vector and "
+            "matrix peaks are both 78.6 TFLOP/s; this kernel issues no MFMA)",
+            "achieved": achieved / 1e12,
             "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
             "traffic": (HEADLINE_TRAFFIC_BYTES if args.variant == "fp64fast_persistent"
                         and n == 128 ** 3 else None),
             "traffic_unit": "bytes per launch (calibrated PMC FETCH_SIZE + WRITE_SIZE)",
             "flops_per_launch": flops, "F_sweep": F_SWEEP,
+            "F_sweep_components": sweep_ops, "F_sweep_source": "dengo_amd.flops.sweep_flops "
+            "on the primordial network's sympy forms",
             "sweeps_per_launch": sweeps}
 
     workloads = {}
@@ -423,7 +464,8 @@ def main():
                         "dtf=1e12 s" % args.synthetic_cells,
             "variants": network_leg("synthetic", syn, xs, rts, ats,
                                     [("fp64fast_persistent", runtime.FP64_FAST)], T=Ts,
-                                    flops_per_sweep=F_SWEEP_DENSE,
+                                    flops_per_sweep=flopcount.dense_sweep_flops(
+                                        net.n, net.nrxn)["total"],
                                     ordered=() if args.no_cost_ordered
                                     else ("fp64fast_persistent",),
                                     density_ordered=() if args.no_cost_ordered
@@ -446,13 +488,21 @@ def main():
         els = []
         for _ in range(max(1, args.cpu_repeats)):
             t0 = time.perf_counter()
-            oracle.prim_evolve(inp[:m], rtol[:m], atol[:m], tab, 1e12, nthreads=threads)
+            ref = oracle.prim_evolve(inp[:m], rtol[:m], atol[:m], tab, 1e12, nthreads=threads)
             els.append(time.perf_counter() - t0)
         el = float(np.median(els))
-        m1 = min(128, m)
+        m1 = m
         t0 = time.perf_counter()
         oracle.prim_evolve(inp[:m1], rtol[:m1], atol[:m1], tab, 1e12, nthreads=1)
         el1 = time.perf_counter() - t0
+        # every GPU run against the reference's own arithmetic (the libm
+        # oracle) on these cells
+        vs_libm = {"vs": "CPU oracle with libm's log (the reference's arithmetic), first %d "
+                         "cells" % m}
+        for label in [k for k in heads if not k.endswith("_status")]:
+            vs_libm[label] = error_stats(heads[label], heads[label + "_status"], ref["state"],
+                                         ref["status"], atol[:m], rtol[:m], 9)
+        parity["vs_libm_oracle"] = vs_libm
         model = "unknown"
         try:
             with open("/proc/cpuinfo") as f:
@@ -460,12 +510,12 @@ def main():
         except (OSError, StopIteration):
             pass
         cpu = {"value": m / el, "unit": "cell-solves/s", "cores": threads, "kind": "port",
-               "sample": "first %d of the %d cells (same seeded inputs), CPU FP64 oracle, "
-                         "OpenMP dynamic schedule, median of %d runs of %.1f s" % (
-                             m, n, len(els), el),
+               "sample": "first %d of the %d cells (same seeded inputs), CPU FP64 oracle "
+                         "(gcc -O3 -march=x86-64-v3, built in the build container), OpenMP "
+                         "dynamic schedule, median of %d runs of %.1f s" % (m, n, len(els), el),
                "cpu_model": model,
                "single_core": {"value": m1 / el1, "unit": "cell-solves/s", "cores": 1,
-                               "sample": "first %d cells, %.1f s" % (m1, el1)}}
+                               "sample": "the same %d cells, one run of %.1f s" % (m1, el1)}}
 
     out = {
         "metric": "cell-solves/s (primordial BE solve, dtf=1e12 s)",

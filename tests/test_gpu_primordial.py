@@ -25,6 +25,16 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
 KEYS = ("state", "atol", "status", "nsub", "nrej", "nsweep", "T")
 
 
+def major_rel(x, ref):
+    """Per cell, the largest relative error over the energy and the species
+    above 1e-10 of the cell's species mass (relative error is ill-posed for
+    species at the 1e-25 floor)."""
+    mass = ref[:, :9].sum(axis=1, keepdims=True)
+    big = np.concatenate([ref[:, :9] > 1e-10 * mass, np.ones((ref.shape[0], 1), bool)], axis=1)
+    rel = np.abs(x - ref) / np.maximum(np.abs(ref), 1e-300)
+    return np.where(big, rel, 0.0).max(axis=1)
+
+
 def _same(a, b, keys=KEYS):
     for k in keys:
         assert np.array_equal(a[k], b[k]), k
@@ -51,16 +61,13 @@ def test_fp64_bitwise_golden(solver, schedule):
 
 
 def test_fp64_vs_libm_oracle(solver):
+    # the libm oracle is the reference's arithmetic; the faithful kernel
+    # differs from it only in the last bit of ln T
     g = np.load(GOLD)
     r = solver.evolve(g["input"], g["rtol"], g["atol"], float(g["dtf"]))
-    done = (g["libm_status"] == 0) & (r["status"] == 0)
-    assert (r["status"] == g["libm_status"]).mean() >= 0.95
-    rel = np.abs(r["state"][done] - g["libm_state"][done]) / np.abs(g["libm_state"][done])
-    # cells whose substep sequence matches the libm run agree to rounding
-    same_path = (r["nsub"] == g["libm_nsub"]) & (r["nrej"] == g["libm_nrej"])
-    sp = same_path[done]
-    assert sp.mean() >= 0.9
-    assert rel[sp].max() < 1e-6
+    assert np.array_equal(r["status"], g["libm_status"])
+    done = g["libm_status"] == 0
+    assert major_rel(r["state"][done], g["libm_state"][done]).max() < 1e-9
 
 
 @pytest.mark.parametrize("ncells", [1, 63, 65, 300])
@@ -200,10 +207,10 @@ def test_fp64_fast_vs_oracle(solver, schedule):
                       precision=runtime.FP64_FAST, schedule=schedule)
     assert np.array_equal(r["status"], g["libm_status"])
     done = g["libm_status"] == 0
-    rel = np.abs(r["state"][done] - g["libm_state"][done]) / np.abs(g["libm_state"][done])
+    # every finished cell, whatever its substep path
+    assert major_rel(r["state"][done], g["libm_state"][done]).max() < 1e-6
     same = ((r["nsub"] == g["libm_nsub"]) & (r["nrej"] == g["libm_nrej"]))[done]
     assert same.mean() >= 0.9
-    assert rel[same].max() < 1e-6
     assert np.all(np.isfinite(r["state"]))
 
 
@@ -227,10 +234,11 @@ def test_mixed_vs_oracle(solver, schedule):
     assert np.all(ref[diff] == 2) and np.all(r["status"][diff] == 1)
     assert np.array_equal(r["status"][ref != 2], ref[ref != 2])
     done = (ref == 0) & (r["status"] == 0)
-    rel = np.abs(r["state"][done] - g["libm_state"][done]) / np.abs(g["libm_state"][done])
+    assert np.array_equal(done, ref == 0)         # no finished cell is lost
+    # every finished cell, whatever its substep path
+    assert major_rel(r["state"][done], g["libm_state"][done]).max() < 1e-6
     same = ((r["nsub"] == g["libm_nsub"]) & (r["nrej"] == g["libm_nrej"]))[done]
     assert same.mean() >= 0.9
-    assert rel[same].max() < 1e-6
 
 
 def test_config1_16cube_bitwise(solver, prim_tables):
