@@ -302,6 +302,18 @@ struct FastTraits<PrimNet> {
     // exchanges no lane chose moves this kernel's register allocation into
     // scratch and costs it about 10%
     static constexpr bool kSkipExchanges32 = false;
+    // Cells with a species near the 1e-25 floor of the substep loop are the
+    // ones the reference runs to the substep cap: which substeps it rejects
+    // is decided by the rounding of its FP64 LU (error relative to the
+    // largest component), which a componentwise-accurate refined solve does
+    // not reproduce.  The mixed sweep solves them with the FP64 block LU.
+    static DENGO_D bool fp64_solve(const double (&u)[prim::N], const prim::Env &)
+    {
+        bool near = false;
+#pragma unroll
+        for (int i = 0; i < prim::GE; ++i) near = near || (u[i] < 1e-24);
+        return near;
+    }
     static DENGO_D bool rhs(const double (&u)[prim::N], double (&f)[prim::N],
                             double (&rs)[prim::NTAB], double &T, double &rmd,
                             const prim::Env &env)
@@ -363,7 +375,11 @@ DENGO_D int sweep_fast(Cell<Net> &c, const KParams<Net> &p)
     FT::jac(c.u, rs, T, rmd, c.sc, dtinv, p.env, A, age);
     double y[M];
     int piv[M];
-    if constexpr (sizeof(LUT) == sizeof(double)) {
+    // the FP64 block LU serves the FP64 sweep and the cells the network's
+    // gate sends to it from the mixed sweep
+    bool use64 = true;
+    if constexpr (sizeof(LUT) != sizeof(double)) use64 = FT::fp64_solve(c.u, p.env);
+    if (use64) {
         double rp[M];
 #pragma unroll
         for (int i = 0; i < M; ++i) y[i] = g[i];

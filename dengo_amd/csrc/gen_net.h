@@ -185,6 +185,10 @@ struct FastTraits<GenNet<G>> {
     static constexpr int M = Net::M;
     static constexpr const LUPlan<M> &plan = Net::plan;
     static constexpr bool kSkipExchanges32 = true;
+    static DENGO_D bool fp64_solve(const double (&)[Net::N], const typename Net::Env &)
+    {
+        return false;
+    }
     static DENGO_D bool rhs(const double (&u)[Net::N], double (&f)[Net::N],
                             double (&rs)[Net::NTAB], double &T, double &rmd,
                             const typename Net::Env &env)

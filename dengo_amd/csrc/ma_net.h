@@ -293,6 +293,10 @@ struct FastTraits<MANet<D>> {
     // the FP32 factorization skips candidate-row exchanges no lane of the
     // wave chose (one uniform branch per candidate; same pivots)
     static constexpr bool kSkipExchanges32 = true;
+    static DENGO_D bool fp64_solve(const double (&)[Net::N], const typename Net::Env &)
+    {
+        return false;
+    }
     static DENGO_D bool rhs(const double (&u)[Net::N], double (&f)[Net::N],
                             double (&rs)[Net::NTAB], double &T, double &rmd,
                             const typename Net::Env &env)
