@@ -70,6 +70,10 @@ def build_solver(output_dir, solver_name, verbose=False):
 
 
 GENERATED = os.path.join(ROOT, "tests", "generated")
+# the oxygen datasets of the reference's input files (tests/golden/make_ion_data.py)
+ION_DATA = os.path.join(ROOT, "tests", "golden", "ion_data")
+# the hydrogen-helium lines of the reference's UMIST table (tests/golden/make_rate12_subset.py)
+RATE12_HHE = os.path.join(ROOT, "tests", "golden", "rate12_hhe.txt")
 
 
 def generated_example_networks():
@@ -78,14 +82,20 @@ def generated_example_networks():
     primordial network through the generator instead of the hand-written
     device code; primphoto, the primordial species with a reduced reaction
     and cooling set plus photoionisation and photoheating on redshift
-    tables."""
+    tables; oxydata, O I - O IX with the reference's Gnat & Ferland cooling and
+    HM12 photoionisation/photoheating tables (data fixtures); umist, the
+    hydrogen-helium reactions of the UMIST RATE12 table (its species sort after
+    the energy term)."""
     if ROOT not in sys.path:
         sys.path.insert(0, ROOT)
     import numpy as np
-    from dengo_amd.networks import ion_network, primordial_network, primordial_photo_network
+    from dengo_amd.networks import (ion_network, oxygen_data_network, primordial_network,
+                                    primordial_photo_network, umist_network)
     with np.errstate(all="ignore"):
         return [("carbon", ion_network("C", seed=9)), ("primgen", primordial_network()),
-                ("primphoto", primordial_photo_network())]
+                ("primphoto", primordial_photo_network()),
+                ("oxydata", oxygen_data_network(ION_DATA)),
+                ("umist", umist_network(RATE12_HHE))]
 
 
 def build_generated_examples():

@@ -150,7 +150,8 @@ def render_data_header(net, solver_name, single=False):
     w("/* Batched RHS f[dims][N], Jacobian J[dims][N][N] (column-major per cell, may")
     w("   be NULL), temperature T[dims] and interpolated tables rates[dims][NTABLES]")
     w("   (may be NULL) for number-density states on host memory; returns the number")
-    w("   of cells whose RHS saw a negative value, or a negative error code. */")
+    w("   of cells whose RHS saw a negative value (their f and J are zero), or a")
+    w("   negative error code. */")
     w("long long dengo_%s_eval(%s_data *data, const double *u, long long dims, double z," % (n, n))
     w("        double *f, double *J, double *T, double *rates);")
     w("/* BE_chem_solve(calculate_rhs_%s, calculate_jacobian_%s, ...) as the Cython" % (n, n))
@@ -404,12 +405,22 @@ def _monomial(term, net, table_syms, de_name="de"):
     return float(coef), table, species, depow
 
 
+def internal_species(net):
+    """The network's unknowns in the device code's order: the solver order
+    (sorted names) with the energy term moved last.  (ext[k] = the position
+    in the caller's rows of internal unknown k.)"""
+    sp = net.ordered_species()
+    ge = net.energy_term
+    internal = [s for s in sp if s != ge] + [ge]
+    return internal, [sp.index(s) for s in internal]
+
+
 def _cooling_monomials(net):
     """(table index, species index or -1, de power, coefficient) of every
     cooling term; raises ValueError unless each is a monomial
     coef * table * [species] * de^p with p <= 2."""
     import sympy
-    species = net.species_list()
+    species = [s.name for s in internal_species(net)[0]]
     idx = {s: i for i, s in enumerate(species)}
     tidx = {t: i for i, t in enumerate(net.table_names())}
     cool = []
@@ -429,12 +440,12 @@ def render_device_network(net, name):
     coef * table * species * de^p (ion-by-ion networks), consumed by the
     generic device network MANet (dengo_amd/csrc/ma_net.h)."""
     import sympy
-    species = net.species_list()
+    internal, ext = internal_species(net)
+    species = [s.name for s in internal]
     idx = {s: i for i, s in enumerate(species)}
     tables = net.table_names()
     tidx = {t: i for i, t in enumerate(tables)}
     n = len(species)
-    assert species[-1] == net.energy_term.name, "energy must sort last"
     rx = []
     for r in net.ordered_reactions():
         slots = []
@@ -447,15 +458,15 @@ def render_device_network(net, name):
             raise ValueError("reaction %s: at most 3 reactants and 4 products" % r.name)
         rx.append((tidx[r.name], slots, sorted(prods.items())))
     cool = _cooling_monomials(net)
-    W = [float(s.weight) for s in net.ordered_species()]
-    FE = [float(getattr(s, "free_electrons", 0.0)) for s in net.ordered_species()]
+    W = [float(s.weight) for s in internal]
+    FE = [float(getattr(s, "free_electrons", 0.0)) for s in internal]
     H2 = ["true" if s in ("H2I", "H2II") else "false" for s in species]
 
     def arr(v):
         return "{" + ", ".join(repr(x) for x in v) + "}"
 
     L = ["// Generated by dengo_amd.codegen from the %s network: do not edit." % name,
-         "// Species (solver order): %s" % " ".join(species),
+         "// Species (device order, energy last): %s" % " ".join(species),
          "// Tables (struct order): %s" % " ".join(tables),
          "#pragma once",
          '#include "ma_net.h"',
@@ -466,6 +477,8 @@ def render_device_network(net, name):
          "    static constexpr int N = %d, NTAB = %d, NRXN = %d, NCOOL = %d;" % (
              n, len(tables), len(rx), len(cool)),
          "    static constexpr int DE = %d, GE = %d;" % (idx["de"], n - 1),
+         "    // position of each unknown in the caller's rows (the solver's sorted order)",
+         "    static constexpr int EXT[N] = %s;" % arr(ext),
          "    static constexpr double W[N] = %s;" % arr(W),
          "    static constexpr double FREE_E[N] = %s;" % arr(FE),
          "    static constexpr bool H2G[N] = {%s};" % ", ".join(H2),
@@ -523,7 +536,7 @@ def gen_network_forms(net):
     dict {(row, col): expr} of the nonzero entries, columns other than the
     energy)."""
     import sympy
-    sp = net.ordered_species()
+    sp = internal_species(net)[0]
     syms = [s.symbol for s in sp]
     ge = net.energy_term
     rates = [r.coeff_sym for r in net.ordered_reactions()]
@@ -550,9 +563,9 @@ def render_gen_network(net, name):
     the species layout, redshift-table slots and the species block's
     structural nonzeros, for GenNet (dengo_amd/csrc/gen_net.h)."""
     import sympy
-    species = net.species_list()
+    internal, ext = internal_species(net)
+    species = [s.name for s in internal]
     n = len(species)
-    assert species[-1] == net.energy_term.name, "energy must sort last"
     syms, rates, rhs, jac = gen_network_forms(net)
     names = {str(x): "u[%d]" % i for i, x in enumerate(syms)}
     names.update({str(r): "rs[%d]" % t for t, r in enumerate(rates)})
@@ -587,8 +600,8 @@ def render_gen_network(net, name):
             out.append("        %s = %s;" % (tgt, pr.doprint(e)))
         return out
 
-    W = [float(s.weight) for s in net.ordered_species()]
-    FE = [float(getattr(s, "free_electrons", 0.0)) for s in net.ordered_species()]
+    W = [float(s.weight) for s in internal]
+    FE = [float(getattr(s, "free_electrons", 0.0)) for s in internal]
     H2 = ["true" if s in ("H2I", "H2II") else "false" for s in species]
     m = n - 1
     jnz = [["true" if (i, j) in jac else "false" for j in range(m)] for i in range(m)]
@@ -597,7 +610,7 @@ def render_gen_network(net, name):
         return "{" + ", ".join(repr(x) for x in v) + "}"
 
     L = ["// Generated by dengo_amd.codegen from the %s network: do not edit." % name,
-         "// Species (solver order): %s" % " ".join(species),
+         "// Species (device order, energy last): %s" % " ".join(species),
          "// Tables (struct order): %s" % " ".join(net.table_names()),
          "// rhs: species_total(s) per species row, the cooling sum in the energy row",
          "// (dengo/chemical_network.py:118-151); jac: their derivatives by each species,",
@@ -614,6 +627,8 @@ def render_gen_network(net, name):
          "struct Gen {",
          "    static constexpr int N = %d, NTAB = %d, DE = %d, GE = %d;" % (
              n, len(rates), de, n - 1),
+         "    // position of each unknown in the caller's rows (the solver's sorted order)",
+         "    static constexpr int EXT[N] = %s;" % arr(ext),
          "    static constexpr double W[N] = %s;" % arr(W),
          "    static constexpr double FREE_E[N] = %s;" % arr(FE),
          "    static constexpr bool H2G[N] = {%s};" % ", ".join(H2),

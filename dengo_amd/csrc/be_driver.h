@@ -27,6 +27,9 @@ struct PrimNet {
     static DENGO_HD double free_e(int k) { return prim::FREE_E[k]; }
     // redshift table slot of table t (-1: a temperature table)
     static constexpr int ztab(int) { return -1; }
+    // position in the caller's rows of internal unknown k (the network works
+    // with the energy last; the rows keep the reference's sorted order)
+    static constexpr int ext(int k) { return k; }
     static DENGO_D double temperature(const double (&u)[N], const TableView &tb)
     {
         return prim::temperature(u, tb);
@@ -91,9 +94,9 @@ DENGO_D void begin_cell(Cell<Net> &c, long long id, const KParams<Net> &p)
     const double *rt = p.rtol + id * N;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        c.u[k] = in[k];
-        c.atol[k] = at[k];
-        c.rtol[k] = rt[k];
+        c.u[k] = in[Net::ext(k)];
+        c.atol[k] = at[Net::ext(k)];
+        c.rtol[k] = rt[Net::ext(k)];
     }
     if (!p.input_is_number) {
 #pragma unroll
@@ -269,8 +272,8 @@ DENGO_D void finish_cell(Cell<Net> &c, const KParams<Net> &p)
     double *at = p.atol + c.id * N;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        out[k] = c.u[k];
-        at[k] = c.atol[k];
+        out[Net::ext(k)] = c.u[k];
+        at[Net::ext(k)] = c.atol[k];
     }
     if (p.status) p.status[c.id] = c.status;
     if (p.nsub) p.nsub[c.id] = c.siter;

@@ -115,7 +115,7 @@ __global__ void __launch_bounds__(64) k_eval(const double *u_in, long long n,
     double u[N], f[N], rs[NT], J[N * N];
     double T;
 #pragma unroll
-    for (int k = 0; k < N; ++k) u[k] = u_in[id * N + k];
+    for (int k = 0; k < N; ++k) u[k] = u_in[id * N + Net::ext(k)];
     const bool ok = Net::rhs(u, f, rs, T, env);
     if (T_out) T_out[id] = T;
     if (rates_out)
@@ -124,11 +124,14 @@ __global__ void __launch_bounds__(64) k_eval(const double *u_in, long long n,
     neg_out[id] = ok ? 0 : 1;
     if (!ok) return;
 #pragma unroll
-    for (int k = 0; k < N; ++k) f_out[id * N + k] = f[k];
+    for (int k = 0; k < N; ++k) f_out[id * N + Net::ext(k)] = f[k];
     if (J_out) {
         Net::jacobian(u, rs, T, env, J);
 #pragma unroll
-        for (int k = 0; k < N * N; ++k) J_out[id * N * N + k] = J[k];
+        for (int col = 0; col < N; ++col)
+#pragma unroll
+            for (int row = 0; row < N; ++row)
+                J_out[id * N * N + Net::ext(col) * N + Net::ext(row)] = J[col * N + row];
     }
 }
 
@@ -151,12 +154,13 @@ __global__ void __launch_bounds__(64) k_strip_sweep(const double *u, double *u_n
     Cell<Net> c;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        c.u[k] = u[id * N + k];
-        c.u0[k] = u0[id * N + k];
-        c.sc[k] = sc[id * N + k];
-        c.inv[k] = inv[id * N + k];
-        c.atol[k] = at[id * N + k];
-        c.rtol[k] = rt[id * N + k];
+        const long long e = id * N + Net::ext(k);
+        c.u[k] = u[e];
+        c.u0[k] = u0[e];
+        c.sc[k] = sc[e];
+        c.inv[k] = inv[e];
+        c.atol[k] = at[e];
+        c.rtol[k] = rt[e];
     }
     c.dt = dt;
     c.sweep = 0;
@@ -168,7 +172,7 @@ __global__ void __launch_bounds__(64) k_strip_sweep(const double *u, double *u_n
     const int r = sweep_fp64(c, p, kind);
     T_out[id] = c.T;
 #pragma unroll
-    for (int k = 0; k < N; ++k) u_new[id * N + k] = c.u[k];
+    for (int k = 0; k < N; ++k) u_new[id * N + Net::ext(k)] = c.u[k];
     code[id] = r == SW_FAILED ? 2 + kind : (r == SW_CONVERGED ? 0 : 1);
 }
 
@@ -185,7 +189,7 @@ __global__ void __launch_bounds__(64) k_temperature_from_mass(const double *in, 
     double u[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        u[k] = in[id * N + k];
+        u[k] = in[id * N + Net::ext(k)];
         if (k != Net::GE) u[k] /= Net::weight(k) * 1.67e-24;
     }
     TableView tb;
@@ -1120,7 +1124,10 @@ struct Host {
         if (T) DENGO_HIP_OK(hipMalloc(&dT, dims * sizeof(double)));
         if (rates) DENGO_HIP_OK(hipMalloc(&dr, (size_t)dims * NTAB * sizeof(double)));
         DENGO_HIP_OK(hipMemcpy(du, u, bu, hipMemcpyHostToDevice));
+        // a cell whose RHS sees a negative value gets neither f nor J: both
+        // stay zero
         DENGO_HIP_OK(hipMemset(df, 0, bu));
+        if (dJ) DENGO_HIP_OK(hipMemset(dJ, 0, bu * N));
         const long long blocks = (dims + 63) / 64;
         hipLaunchKernelGGL(k_eval<Net>, dim3((unsigned)blocks), dim3(64), 0, 0, du, dims,
                            env(d, s, z), df, dJ, dT, dr, dn);
@@ -1275,8 +1282,8 @@ struct Host {
         for (int i = 0; i < nstrip; ++i) {
             double *c = input + (size_t)i * N;
             for (int k = 0; k < N; ++k)
-                if (k != Net::GE && k != Net::DE) total_e += c[k] * Net::free_e(k);
-            c[Net::DE] = total_e;
+                if (k != Net::GE && k != Net::DE) total_e += c[Net::ext(k)] * Net::free_e(k);
+            c[Net::ext(Net::DE)] = total_e;
         }
     }
 

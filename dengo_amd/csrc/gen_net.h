@@ -43,6 +43,7 @@ struct GenNet {
     static DENGO_D double weight(int k) { return G::W[k]; }
     static DENGO_HD double free_e(int k) { return G::FREE_E[k]; }
     static constexpr int ztab(int t) { return G::ZTAB[t]; }
+    static constexpr int ext(int k) { return G::EXT[k]; }
 
     static DENGO_D double temperature(const double (&u)[N], const TableView &tb)
     {

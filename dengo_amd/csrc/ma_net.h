@@ -67,6 +67,7 @@ struct MANet {
     static DENGO_D double weight(int k) { return D::W[k]; }
     static DENGO_HD double free_e(int k) { return D::FREE_E[k]; }
     static constexpr int ztab(int) { return -1; }
+    static constexpr int ext(int k) { return D::EXT[k]; }
 
     static DENGO_D double temperature(const double (&u)[N], const TableView &tb)
     {

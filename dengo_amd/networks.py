@@ -129,3 +129,64 @@ def primordial_photo_network(T_bounds=(1.0, 1e9), n_bins=1024, z_bounds=(0.0, 10
     net.init_redshift(z_bounds, n_z_bins)
     assert tuple(net.species_list()) == PRIMORDIAL_SPECIES, net.species_list()
     return net
+
+
+def oxygen_data_network(data_dir, seed=3, T_bounds=(1e2, 1e9), n_bins=1024,
+                        z_bounds=(0.0, 9.0), n_z_bins=100, photo_background="HM12",
+                        stop_time=1e12):
+    """O I - O IX with the reference's data tables where they exist: Gnat &
+    Ferland ion cooling (input/cooling) on 1024 log-T bins over 1e2-1e9 K and
+    the HM12 photoionisation and photoheating rates (input/photo*) on 100
+    redshift bins over z = 0-9; the collisional ionisation and recombination
+    rates, which the reference takes from ChiantiPy, are the seeded synthetic
+    ones of ion_network.  data_dir: the directory holding input/ (the
+    reference's tree, or tests/golden/ion_data)."""
+    from .ion_by_ion import ATOMIC_NUMBER, setup_ionization, synthetic_ion_tables
+    Z = ATOMIC_NUMBER["O"]
+    fns = {k: v for k, v in synthetic_ion_tables("O", seed).items() if not k.endswith("_c")}
+    setup_ionization("O", fns, cooling=True, photo_background=photo_background,
+                     data_dir=data_dir)
+    net = ChemicalNetwork(stop_time=stop_time)
+    for k in range(1, Z + 2):
+        net.add_cooling("O_%d_c" % k)
+        if k < Z + 1:
+            net.add_cooling("O_%d_ph" % k)
+            net.add_reaction("O_%d_i" % k)
+            net.add_reaction("O_%d_pi" % k)
+        if k > 1:
+            net.add_reaction("O_%d_r" % k)
+    net.init_temperature(T_bounds, n_bins)
+    net.init_redshift(z_bounds, n_z_bins)
+    assert tuple(net.species_list()) == OXYGEN_SPECIES, net.species_list()
+    return net
+
+
+UMIST_HHE = ("H", "H2", "H+", "H2+", "H3+", "H-", "e-", "He", "He+", "HeH+")
+
+
+def umist_weight(symbol):
+    """Mass of a UMIST species of hydrogen and helium in atomic units."""
+    import re
+    from .chemistry_constants import ATOMIC_WEIGHT
+    base = symbol.replace("+", "").replace("-", "")
+    w = 0.0
+    for el, cnt in re.findall(r"(He|H)(\d*)", base):
+        w += ATOMIC_WEIGHT[el] * (int(cnt) if cnt else 1)
+    return w
+
+
+def umist_network(rate12_path, symbols=UMIST_HHE, T_bounds=(10.0, 41000.0), n_bins=1024,
+                  stop_time=1e12):
+    """A network of the UMIST RATE12 reactions among `symbols`
+    (dengo/get_rates.py): species us_<name>_<charge + 1>, electrons as de, no
+    cooling actions; the temperature grid spans RATE12's usual validity
+    range."""
+    from .umist import setup_umist_reactions, setup_umist_species
+    for s in symbols:
+        setup_umist_species(s, umist_weight(s))
+    names = setup_umist_reactions(symbols, rate12_path)
+    net = ChemicalNetwork(stop_time=stop_time)
+    for n in names:
+        net.add_reaction(n)
+    net.init_temperature(T_bounds, n_bins)
+    return net

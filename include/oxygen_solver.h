@@ -199,7 +199,8 @@ double dengo_oxygen_evolve_host(oxygen_data *data, const dengo_options *opt, dou
 /* Batched RHS f[dims][N], Jacobian J[dims][N][N] (column-major per cell, may
    be NULL), temperature T[dims] and interpolated tables rates[dims][NTABLES]
    (may be NULL) for number-density states on host memory; returns the number
-   of cells whose RHS saw a negative value, or a negative error code. */
+   of cells whose RHS saw a negative value (their f and J are zero), or a
+   negative error code. */
 long long dengo_oxygen_eval(oxygen_data *data, const double *u, long long dims, double z,
         double *f, double *J, double *T, double *rates);
 /* BE_chem_solve(calculate_rhs_oxygen, calculate_jacobian_oxygen, ...) as the Cython

@@ -49,7 +49,7 @@ int orc_gauss_elim(double *A, double *x, double *b, int n);
 #define ORC_MAXTAB 64
 #define ORC_MAXRXN 64
 typedef struct {
-    int table, nslot, slot[3], nprod, pc[2], ps[2];
+    int table, nslot, slot[3], nprod, pc[4], ps[4];   /* up to 4 distinct products */
 } orc_rxn;
 typedef struct {
     int table, species, depow;

@@ -153,7 +153,7 @@ def prim_evolve(inp, rtol, atol, tables, dtf, dt=-1.0, z=-1.0, T_bounds=(1.0, 1e
 # ---------------------------------------------------------------------------
 class _Rxn(ctypes.Structure):
     _fields_ = [("table", ctypes.c_int), ("nslot", ctypes.c_int), ("slot", ctypes.c_int * 3),
-                ("nprod", ctypes.c_int), ("pc", ctypes.c_int * 2), ("ps", ctypes.c_int * 2)]
+                ("nprod", ctypes.c_int), ("pc", ctypes.c_int * 4), ("ps", ctypes.c_int * 4)]
 
 
 class _Cool(ctypes.Structure):
@@ -203,6 +203,65 @@ def ion_chain_net(Z, weight):
     net.ncool = nion
     net.ntab = len(names) + nion
     return net
+
+
+def ma_net_from_network(network):
+    """orc_ma_net of a mass-action network without cooling terms or with
+    -c * X * de terms (dengo_amd ChemicalNetwork), unknowns in the internal
+    order (energy last); .ext holds each unknown's position in the caller's
+    rows."""
+    sp = network.ordered_species()
+    ge = network.energy_term
+    internal = [s for s in sp if s != ge] + [ge]
+    idx = {s.name: i for i, s in enumerate(internal)}
+    net = MANet()
+    net.n, net.de = len(internal), idx["de"]
+    for k, s in enumerate(internal):
+        net.W[k] = float(s.weight)
+        net.free_e[k] = float(getattr(s, "free_electrons", 0.0))
+        net.h2gamma[k] = 1 if s.name in ("H2I", "H2II") else 0
+    tabs = network.table_names()
+    for t, r in enumerate(network.ordered_reactions()):
+        x = net.rxn[t]
+        slots = [idx[s.name] for c, s in r.left_side for _ in range(int(c))]
+        x.table, x.nslot = tabs.index(r.name), len(slots)
+        for k, v in enumerate(slots):
+            x.slot[k] = v
+        prods = {}
+        for c, s in r.right_side:
+            prods[idx[s.name]] = prods.get(idx[s.name], 0) + int(c)
+        items = sorted(prods.items())
+        if len(items) > 4:
+            raise ValueError("reaction %s has more than four distinct products" % r.name)
+        x.nprod = len(items)
+        for k, (sv, c) in enumerate(items):
+            x.ps[k], x.pc[k] = sv, c
+    net.nrxn = len(network.reactions)
+    if network.cooling_actions:
+        raise ValueError("ma_net_from_network builds networks without cooling actions")
+    net.ncool = 0
+    net.ntab = len(tabs)
+    net.ext = np.array([sp.index(s) for s in internal])
+    return net
+
+
+def ma_eval_external(net, u, tables, T_bounds, flags=0):
+    """ma_eval for rows in the caller's order (net from ma_net_from_network)."""
+    ext = net.ext
+    f, J, T, rates, nneg = ma_eval(net, _to_internal(u, ext), tables, T_bounds, flags)
+    Je = np.empty_like(J)
+    Je[:, ext[:, None], ext[None, :]] = J
+    return _to_external(f, ext), Je, T, rates, nneg
+
+
+def ma_evolve_external(net, inp, rtol, atol, tables, dtf, T_bounds, **kw):
+    """ma_evolve for rows in the caller's order."""
+    ext = net.ext
+    r = ma_evolve(net, _to_internal(inp, ext), _to_internal(rtol, ext), _to_internal(atol, ext),
+                  tables, dtf, T_bounds, **kw)
+    r["state"] = _to_external(r["state"], ext)
+    r["atol"] = _to_external(r["atol"], ext)
+    return r
 
 
 def _ma_lib():
@@ -391,12 +450,13 @@ void gen_jac(const double *u, const double *rs, double T, double z, double *J)
            *reinterpret_cast<double (*)[G::N * G::N]>(J));
 }
 void gen_layout(int *n, int *ntab, int *de, double *W, double *fe, int *h2, int *zslot,
-                double *zb, int *nz)
+                double *zb, int *nz, int *ext)
 {
     *n = G::N;
     *ntab = G::NTAB;
     *de = G::DE;
     for (int k = 0; k < G::N; ++k) {
+        ext[k] = G::EXT[k];
         W[k] = G::W[k];
         fe[k] = G::FREE_E[k];
         h2[k] = G::H2G[k] ? 1 : 0;
@@ -438,8 +498,10 @@ def gen_net(name):
         lib = ctypes.CDLL(path)
         net = GenNet()
         n, ntab, de, nz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ext = (ctypes.c_int * 16)()
         lib.gen_layout(ctypes.byref(n), ctypes.byref(ntab), ctypes.byref(de), net.W,
-                       net.free_e, net.h2gamma, net.zslot, net.z_bounds, ctypes.byref(nz))
+                       net.free_e, net.h2gamma, net.zslot, net.z_bounds, ctypes.byref(nz), ext)
+        net.ext = np.array(list(ext)[:n.value])
         net.n, net.ntab, net.de, net.nz = n.value, ntab.value, de.value, nz.value
         net.rhs = ctypes.cast(lib.gen_rhs, ctypes.c_void_p).value
         net.jac = ctypes.cast(lib.gen_jac, ctypes.c_void_p).value
@@ -474,8 +536,22 @@ def z_rate(r, z_bounds, z):
     return _gen_lib().orc_z_rate(_ptr(r), r.size, z_bounds[0], z_bounds[1], z)
 
 
+def _to_internal(a, ext):
+    """Rows in the caller's order -> the network's internal order (energy last)."""
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64)[:, ext])
+
+
+def _to_external(a, ext):
+    out = np.empty_like(a)
+    out[:, ext] = a
+    return out
+
+
 def gen_eval(net, u, tables, T_bounds, z=-1.0, flags=0):
-    u = np.ascontiguousarray(u, dtype=np.float64)
+    """(f, J, T, rates, nneg) of a generated network for rows u in the
+    caller's order (J[c, col, row] in that order too)."""
+    ext = net.ext
+    u = _to_internal(u, ext)
     n, N = u.shape
     f = np.zeros((n, N))
     J = np.zeros((n, N, N))
@@ -485,16 +561,19 @@ def gen_eval(net, u, tables, T_bounds, z=-1.0, flags=0):
     nneg = _gen_lib().orc_gen_eval(ctypes.byref(net), _ptr(u), n, z, _ptr(tab), tab.shape[1],
                                    T_bounds[0], T_bounds[1], _ptr(f), _ptr(J), _ptr(T),
                                    _ptr(rates), flags)
-    return f, J, T, rates, nneg
+    Je = np.empty_like(J)
+    Je[:, ext[:, None], ext[None, :]] = J
+    return _to_external(f, ext), Je, T, rates, nneg
 
 
 def gen_evolve(net, inp, rtol, atol, tables, dtf, T_bounds, dt=-1.0, z=-1.0,
                input_is_number=False, nthreads=0, flags=0, max_substeps=50000):
     lib = _gen_lib()
     lib.orc_set_max_substeps(max_substeps)
-    out = np.array(inp, dtype=np.float64, copy=True, order="C")
-    atol_o = np.array(atol, dtype=np.float64, copy=True, order="C")
-    rtol_c = np.array(rtol, dtype=np.float64, copy=True, order="C")
+    ext = net.ext
+    out = _to_internal(inp, ext)
+    atol_o = _to_internal(atol, ext)
+    rtol_c = _to_internal(rtol, ext)
     n = out.shape[0]
     status, nsub, nrej, nsweep = (np.zeros(n, dtype=np.int32) for _ in range(4))
     dt_out, T_out = np.zeros(n), np.zeros(n)
@@ -506,8 +585,9 @@ def gen_evolve(net, inp, rtol, atol, tables, dtf, T_bounds, dt=-1.0, z=-1.0,
         _ptr(status, _ip), _ptr(nsub, _ip), _ptr(nrej, _ip), _ptr(nsweep, _ip), _ptr(dt_out),
         _ptr(T_out), nthreads, flags)
     lib.orc_set_max_substeps(50000)
-    return dict(state=out, atol=atol_o, ttot=ttot, dt=dtc.value, status=status, nsub=nsub,
-                nrej=nrej, nsweep=nsweep, dt_cell=dt_out, T=T_out)
+    return dict(state=_to_external(out, ext), atol=_to_external(atol_o, ext), ttot=ttot,
+                dt=dtc.value, status=status, nsub=nsub, nrej=nrej, nsweep=nsweep, dt_cell=dt_out,
+                T=T_out)
 
 
 class StripSolver(object):

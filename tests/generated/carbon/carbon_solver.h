@@ -175,7 +175,8 @@ double dengo_carbon_evolve_host(carbon_data *data, const dengo_options *opt, dou
 /* Batched RHS f[dims][N], Jacobian J[dims][N][N] (column-major per cell, may
    be NULL), temperature T[dims] and interpolated tables rates[dims][NTABLES]
    (may be NULL) for number-density states on host memory; returns the number
-   of cells whose RHS saw a negative value, or a negative error code. */
+   of cells whose RHS saw a negative value (their f and J are zero), or a
+   negative error code. */
 long long dengo_carbon_eval(carbon_data *data, const double *u, long long dims, double z,
         double *f, double *J, double *T, double *rates);
 /* BE_chem_solve(calculate_rhs_carbon, calculate_jacobian_carbon, ...) as the Cython

@@ -1,0 +1,263 @@
+// Generated by dengo_amd.codegen from the oxydata network: do not edit.
+// Species (device order, energy last): O_1 O_2 O_3 O_4 O_5 O_6 O_7 O_8 O_9 de ge
+// Tables (struct order): O_1_i O_1_pi O_2_i O_2_pi O_2_r O_3_i O_3_pi O_3_r O_4_i O_4_pi O_4_r O_5_i O_5_pi O_5_r O_6_i O_6_pi O_6_r O_7_i O_7_pi O_7_r O_8_i O_8_pi O_8_r O_9_r O_1_c_O_1_c O_1_ph_O_1_ph O_2_c_O_2_c O_2_ph_O_2_ph O_3_c_O_3_c O_3_ph_O_3_ph O_4_c_O_4_c O_4_ph_O_4_ph O_5_c_O_5_c O_5_ph_O_5_ph O_6_c_O_6_c O_6_ph_O_6_ph O_7_c_O_7_c O_7_ph_O_7_ph O_8_c_O_8_c O_8_ph_O_8_ph O_9_c_O_9_c
+// rhs: species_total(s) per species row, the cooling sum in the energy row
+// (dengo/chemical_network.py:118-151); jac: their derivatives by each species,
+// column-major, energy column zero (:153-168).  Common subexpressions by sympy.cse.
+#pragma once
+#include <math.h>
+
+#ifndef DENGO_GEN_FN
+#error "define DENGO_GEN_FN (the function qualifiers) before including this header"
+#endif
+
+namespace dengo {
+namespace net_oxydata {
+struct Gen {
+    static constexpr int N = 11, NTAB = 41, DE = 9, GE = 10;
+    // position of each unknown in the caller's rows (the solver's sorted order)
+    static constexpr int EXT[N] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10};
+    static constexpr double W[N] = {15.9994, 15.9994, 15.9994, 15.9994, 15.9994, 15.9994, 15.9994, 15.9994, 15.9994, 1.0, 1.0};
+    static constexpr double FREE_E[N] = {0.0, 1.0, 2.0, 3.0, 4.0, 5.0, 6.0, 7.0, 8.0, 0.0, 0.0};
+    static constexpr bool H2G[N] = {false, false, false, false, false, false, false, false, false, false, false};
+    // redshift-table slot of each table (-1: a temperature table)
+    static constexpr int ZTAB[NTAB] = {-1, 0, -1, 1, -1, -1, 2, -1, -1, 3, -1, -1, 4, -1, -1, 5, -1, -1, 6, -1, -1, 7, -1, -1, -1, 8, -1, 9, -1, 10, -1, 11, -1, 12, -1, 13, -1, 14, -1, 15, -1};
+    // structural nonzeros of the species block of the Jacobian
+    static constexpr bool JNZ[N - 1][N - 1] = {
+        {true, true, false, false, false, false, false, false, false, true},
+        {true, true, true, false, false, false, false, false, false, true},
+        {false, true, true, true, false, false, false, false, false, true},
+        {false, false, true, true, true, false, false, false, false, true},
+        {false, false, false, true, true, true, false, false, false, true},
+        {false, false, false, false, true, true, true, false, false, true},
+        {false, false, false, false, false, true, true, true, false, true},
+        {false, false, false, false, false, false, true, true, true, true},
+        {false, false, false, false, false, false, false, true, true, true},
+        {true, true, true, true, true, true, true, true, true, true},
+    };
+
+    static DENGO_GEN_FN void rhs(const double (&u)[N], const double (&rs)[NTAB], double T,
+                                 double z, double (&f)[N])
+    {
+        (void)T;
+        (void)z;
+        const double c0 = u[0]*u[9];
+        const double c1 = u[1]*u[9];
+        const double c2 = rs[0]*c0 + rs[1]*u[0] - rs[4]*c1;
+        const double c3 = rs[3]*u[1];
+        const double c4 = u[2]*u[9];
+        const double c5 = rs[7]*c4;
+        const double c6 = rs[2]*c1;
+        const double c7 = rs[6]*u[2];
+        const double c8 = u[3]*u[9];
+        const double c9 = rs[10]*c8;
+        const double c10 = rs[5]*c4;
+        const double c11 = c3 - c5 + c6;
+        const double c12 = rs[9]*u[3];
+        const double c13 = u[4]*u[9];
+        const double c14 = rs[13]*c13;
+        const double c15 = rs[8]*c8;
+        const double c16 = c10 + c7 - c9;
+        const double c17 = rs[12]*u[4];
+        const double c18 = u[5]*u[9];
+        const double c19 = rs[16]*c18;
+        const double c20 = rs[11]*c13;
+        const double c21 = c12 - c14 + c15;
+        const double c22 = rs[15]*u[5];
+        const double c23 = u[6]*u[9];
+        const double c24 = rs[19]*c23;
+        const double c25 = rs[14]*c18;
+        const double c26 = c17 - c19 + c20;
+        const double c27 = rs[18]*u[6];
+        const double c28 = u[7]*u[9];
+        const double c29 = rs[22]*c28;
+        const double c30 = rs[17]*c23;
+        const double c31 = c22 - c24 + c25;
+        const double c32 = rs[21]*u[7];
+        const double c33 = u[8]*u[9];
+        const double c34 = rs[23]*c33;
+        const double c35 = rs[20]*c28;
+        const double c36 = c27 - c29 + c30;
+        const double c37 = c32 - c34 + c35;
+        f[0] = -c2;
+        f[1] = c2 - c3 + c5 - c6;
+        f[2] = -c10 + c11 - c7 + c9;
+        f[3] = -c12 + c14 - c15 + c16;
+        f[4] = -c17 + c19 - c20 + c21;
+        f[5] = -c22 + c24 - c25 + c26;
+        f[6] = -c27 + c29 - c30 + c31;
+        f[7] = -c32 + c34 - c35 + c36;
+        f[8] = c37;
+        f[9] = c11 + c16 + c2 + c21 + c26 + c31 + c36 + c37;
+        f[10] = u[0]*rs[25] - rs[24]*c0 + u[1]*rs[27] - rs[26]*c1 + u[2]*rs[29] - rs[28]*c4 + u[3]*rs[31] - rs[30]*c8 + u[4]*rs[33] - rs[32]*c13 + u[5]*rs[35] - rs[34]*c18 + u[6]*rs[37] - rs[36]*c23 + u[7]*rs[39] - rs[38]*c28 - rs[40]*c33;
+    }
+
+    static DENGO_GEN_FN void jac(const double (&u)[N], const double (&rs)[NTAB], double T,
+                                 double z, double (&J)[N * N])
+    {
+        (void)T;
+        (void)z;
+        const double c0 = rs[0]*u[9] + rs[1];
+        const double c1 = rs[4]*u[9];
+        const double c2 = rs[2]*u[9] + rs[3];
+        const double c3 = rs[7]*u[9];
+        const double c4 = rs[5]*u[9] + rs[6];
+        const double c5 = rs[10]*u[9];
+        const double c6 = rs[8]*u[9] + rs[9];
+        const double c7 = rs[13]*u[9];
+        const double c8 = rs[11]*u[9] + rs[12];
+        const double c9 = rs[16]*u[9];
+        const double c10 = rs[14]*u[9] + rs[15];
+        const double c11 = rs[19]*u[9];
+        const double c12 = rs[17]*u[9] + rs[18];
+        const double c13 = rs[22]*u[9];
+        const double c14 = rs[20]*u[9] + rs[21];
+        const double c15 = rs[23]*u[9];
+        const double c16 = -c15;
+        const double c17 = rs[0]*u[0] - rs[4]*u[1];
+        const double c18 = rs[7]*u[2];
+        const double c19 = rs[2]*u[1];
+        const double c20 = rs[10]*u[3];
+        const double c21 = rs[5]*u[2];
+        const double c22 = -c18 + c19;
+        const double c23 = rs[13]*u[4];
+        const double c24 = rs[8]*u[3];
+        const double c25 = -c20 + c21;
+        const double c26 = rs[16]*u[5];
+        const double c27 = rs[11]*u[4];
+        const double c28 = -c23 + c24;
+        const double c29 = rs[19]*u[6];
+        const double c30 = rs[14]*u[5];
+        const double c31 = -c26 + c27;
+        const double c32 = rs[22]*u[7];
+        const double c33 = rs[17]*u[6];
+        const double c34 = -c29 + c30;
+        const double c35 = rs[23]*u[8];
+        const double c36 = rs[20]*u[7];
+        const double c37 = -c32 + c33;
+        const double c38 = -c35 + c36;
+        J[0] = -c0;
+        J[1] = c0;
+        J[9] = c0;
+        J[10] = -rs[24]*u[9] + rs[25];
+        J[11] = c1;
+        J[12] = -c1 - c2;
+        J[13] = c2;
+        J[20] = -c1 + c2;
+        J[21] = -rs[26]*u[9] + rs[27];
+        J[23] = c3;
+        J[24] = -c3 - c4;
+        J[25] = c4;
+        J[31] = -c3 + c4;
+        J[32] = -rs[28]*u[9] + rs[29];
+        J[35] = c5;
+        J[36] = -c5 - c6;
+        J[37] = c6;
+        J[42] = -c5 + c6;
+        J[43] = -rs[30]*u[9] + rs[31];
+        J[47] = c7;
+        J[48] = -c7 - c8;
+        J[49] = c8;
+        J[53] = -c7 + c8;
+        J[54] = -rs[32]*u[9] + rs[33];
+        J[59] = c9;
+        J[60] = -c10 - c9;
+        J[61] = c10;
+        J[64] = c10 - c9;
+        J[65] = -rs[34]*u[9] + rs[35];
+        J[71] = c11;
+        J[72] = -c11 - c12;
+        J[73] = c12;
+        J[75] = -c11 + c12;
+        J[76] = -rs[36]*u[9] + rs[37];
+        J[83] = c13;
+        J[84] = -c13 - c14;
+        J[85] = c14;
+        J[86] = -c13 + c14;
+        J[87] = -rs[38]*u[9] + rs[39];
+        J[95] = c15;
+        J[96] = c16;
+        J[97] = c16;
+        J[98] = -rs[40]*u[9];
+        J[99] = -c17;
+        J[100] = c17 + c18 - c19;
+        J[101] = c20 - c21 + c22;
+        J[102] = c23 - c24 + c25;
+        J[103] = c26 - c27 + c28;
+        J[104] = c29 - c30 + c31;
+        J[105] = c32 - c33 + c34;
+        J[106] = c35 - c36 + c37;
+        J[107] = c38;
+        J[108] = c17 + c22 + c25 + c28 + c31 + c34 + c37 + c38;
+        J[109] = -u[0]*rs[24] - u[1]*rs[26] - u[2]*rs[28] - u[3]*rs[30] - u[4]*rs[32] - u[5]*rs[34] - u[6]*rs[36] - u[7]*rs[38] - u[8]*rs[40];
+        J[2] = 0.0;
+        J[3] = 0.0;
+        J[4] = 0.0;
+        J[5] = 0.0;
+        J[6] = 0.0;
+        J[7] = 0.0;
+        J[8] = 0.0;
+        J[14] = 0.0;
+        J[15] = 0.0;
+        J[16] = 0.0;
+        J[17] = 0.0;
+        J[18] = 0.0;
+        J[19] = 0.0;
+        J[22] = 0.0;
+        J[26] = 0.0;
+        J[27] = 0.0;
+        J[28] = 0.0;
+        J[29] = 0.0;
+        J[30] = 0.0;
+        J[33] = 0.0;
+        J[34] = 0.0;
+        J[38] = 0.0;
+        J[39] = 0.0;
+        J[40] = 0.0;
+        J[41] = 0.0;
+        J[44] = 0.0;
+        J[45] = 0.0;
+        J[46] = 0.0;
+        J[50] = 0.0;
+        J[51] = 0.0;
+        J[52] = 0.0;
+        J[55] = 0.0;
+        J[56] = 0.0;
+        J[57] = 0.0;
+        J[58] = 0.0;
+        J[62] = 0.0;
+        J[63] = 0.0;
+        J[66] = 0.0;
+        J[67] = 0.0;
+        J[68] = 0.0;
+        J[69] = 0.0;
+        J[70] = 0.0;
+        J[74] = 0.0;
+        J[77] = 0.0;
+        J[78] = 0.0;
+        J[79] = 0.0;
+        J[80] = 0.0;
+        J[81] = 0.0;
+        J[82] = 0.0;
+        J[88] = 0.0;
+        J[89] = 0.0;
+        J[90] = 0.0;
+        J[91] = 0.0;
+        J[92] = 0.0;
+        J[93] = 0.0;
+        J[94] = 0.0;
+        J[110] = 0.0;
+        J[111] = 0.0;
+        J[112] = 0.0;
+        J[113] = 0.0;
+        J[114] = 0.0;
+        J[115] = 0.0;
+        J[116] = 0.0;
+        J[117] = 0.0;
+        J[118] = 0.0;
+        J[119] = 0.0;
+        J[120] = 0.0;
+    }
+};
+}  // namespace net_oxydata
+}  // namespace dengo

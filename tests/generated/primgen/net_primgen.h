@@ -1,5 +1,5 @@
 // Generated by dengo_amd.codegen from the primgen network: do not edit.
-// Species (solver order): H2I H2II HI HII HM HeI HeII HeIII de ge
+// Species (device order, energy last): H2I H2II HI HII HM HeI HeII HeIII de ge
 // Tables (struct order): k01 k02 k03 k04 k05 k06 k07 k08 k09 k10 k11 k12 k13 k14 k15 k16 k17 k18 k19 k21 k22 k23 brem_brem ceHeI_ceHeI ceHeII_ceHeII ceHI_ceHI ciHeI_ciHeI ciHeII_ciHeII ciHeIS_ciHeIS ciHI_ciHI compton_comp gammah_gammah gloverabel08_gael gloverabel08_gaH2 gloverabel08_gaHe gloverabel08_gaHI gloverabel08_gaHp gloverabel08_gphdl gloverabel08_gpldl h2formation_h2mcool h2formation_h2mheat reHeII1_reHeII1 reHeII2_reHeII2 reHeIII_reHeIII reHII_reHII
 // rhs: species_total(s) per species row, the cooling sum in the energy row
 // (dengo/chemical_network.py:118-151); jac: their derivatives by each species,
@@ -15,6 +15,8 @@ namespace dengo {
 namespace net_primgen {
 struct Gen {
     static constexpr int N = 10, NTAB = 45, DE = 8, GE = 9;
+    // position of each unknown in the caller's rows (the solver's sorted order)
+    static constexpr int EXT[N] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9};
     static constexpr double W[N] = {2.0, 2.0, 1.00794, 1.00794, 1.00794, 4.002602, 4.002602, 4.002602, 1.0, 1.0};
     static constexpr double FREE_E[N] = {0.0, 1.0, 0.0, 1.0, -1.0, 0.0, 1.0, 2.0, 0.0, 0.0};
     static constexpr bool H2G[N] = {true, true, false, false, false, false, false, false, false, false};

@@ -279,7 +279,8 @@ double dengo_primgen_evolve_host(primgen_data *data, const dengo_options *opt, d
 /* Batched RHS f[dims][N], Jacobian J[dims][N][N] (column-major per cell, may
    be NULL), temperature T[dims] and interpolated tables rates[dims][NTABLES]
    (may be NULL) for number-density states on host memory; returns the number
-   of cells whose RHS saw a negative value, or a negative error code. */
+   of cells whose RHS saw a negative value (their f and J are zero), or a
+   negative error code. */
 long long dengo_primgen_eval(primgen_data *data, const double *u, long long dims, double z,
         double *f, double *J, double *T, double *rates);
 /* BE_chem_solve(calculate_rhs_primgen, calculate_jacobian_primgen, ...) as the Cython

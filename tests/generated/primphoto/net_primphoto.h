@@ -1,5 +1,5 @@
 // Generated by dengo_amd.codegen from the primphoto network: do not edit.
-// Species (solver order): H2I H2II HI HII HM HeI HeII HeIII de ge
+// Species (device order, energy last): H2I H2II HI HII HM HeI HeII HeIII de ge
 // Tables (struct order): HI_pi k01 k02 k03 k04 k05 k06 k07 k08 k09 k10 brem_brem ciHI_ciHI compton_comp gloverabel08_gael gloverabel08_gaH2 gloverabel08_gaHe gloverabel08_gaHI gloverabel08_gaHp gloverabel08_gphdl gloverabel08_gpldl HI_ph_HI_ph reHII_reHII
 // rhs: species_total(s) per species row, the cooling sum in the energy row
 // (dengo/chemical_network.py:118-151); jac: their derivatives by each species,
@@ -15,6 +15,8 @@ namespace dengo {
 namespace net_primphoto {
 struct Gen {
     static constexpr int N = 10, NTAB = 23, DE = 8, GE = 9;
+    // position of each unknown in the caller's rows (the solver's sorted order)
+    static constexpr int EXT[N] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9};
     static constexpr double W[N] = {2.0, 2.0, 1.00794, 1.00794, 1.00794, 4.002602, 4.002602, 4.002602, 1.0, 1.0};
     static constexpr double FREE_E[N] = {0.0, 1.0, 0.0, 1.0, -1.0, 0.0, 1.0, 2.0, 0.0, 0.0};
     static constexpr bool H2G[N] = {true, true, false, false, false, false, false, false, false, false};

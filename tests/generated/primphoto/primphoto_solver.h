@@ -191,7 +191,8 @@ double dengo_primphoto_evolve_host(primphoto_data *data, const dengo_options *op
 /* Batched RHS f[dims][N], Jacobian J[dims][N][N] (column-major per cell, may
    be NULL), temperature T[dims] and interpolated tables rates[dims][NTABLES]
    (may be NULL) for number-density states on host memory; returns the number
-   of cells whose RHS saw a negative value, or a negative error code. */
+   of cells whose RHS saw a negative value (their f and J are zero), or a
+   negative error code. */
 long long dengo_primphoto_eval(primphoto_data *data, const double *u, long long dims, double z,
         double *f, double *J, double *T, double *rates);
 /* BE_chem_solve(calculate_rhs_primphoto, calculate_jacobian_primphoto, ...) as the Cython

@@ -1,0 +1,215 @@
+/* Drop-in header for the umist network (generated by dengo_amd.codegen).
+   Struct fields and order follow dengo/templates/rates_and_rate_tables.h.template:30-83;
+   entry points follow :85-108 with C linkage. */
+#ifndef UMIST_SOLVER_H
+#define UMIST_SOLVER_H
+
+#ifndef MAX_NCELLS
+#define MAX_NCELLS 1024
+#endif
+#define UMIST_NSPECIES 11
+#define UMIST_NTABLES 18
+#define UMIST_NT 1024
+/* Temperature grid of the tables and <name>_main's dtf
+   (rates_and_rate_tables.C.template:25-26, 110). */
+#define UMIST_T_LO 10.0
+#define UMIST_T_HI 41000.0
+#define UMIST_STOP_TIME 1000000000000.0
+/* Redshift grid of the pi/ph tables (rates_and_rate_tables.C.template:31-36). */
+#define UMIST_NZ 1
+#define UMIST_Z_LO 0.0
+#define UMIST_Z_HI 0.0
+
+typedef struct umist_data_ {
+    double dbin;
+    double idbin;
+    double bounds[2];
+    int nbins;
+    double d_zbin;
+    double id_zbin;
+    double z_bounds[2];
+    int n_zbins;
+    double current_z;
+    double zdef;
+    double dz;
+    double Ts[MAX_NCELLS];
+    double Tdef[MAX_NCELLS];
+    double dT[MAX_NCELLS];
+    double logTs[MAX_NCELLS];
+    double invTs[MAX_NCELLS];
+    double dTs_ge[MAX_NCELLS];
+    double r_us_H2_1_plus_de[1024];
+    double rs_us_H2_1_plus_de[MAX_NCELLS];
+    double drs_us_H2_1_plus_de[MAX_NCELLS];
+    double r_us_H2_1_plus_us_H2_1[1024];
+    double rs_us_H2_1_plus_us_H2_1[MAX_NCELLS];
+    double drs_us_H2_1_plus_us_H2_1[MAX_NCELLS];
+    double r_us_H2_1_plus_us_He_2[1024];
+    double rs_us_H2_1_plus_us_He_2[MAX_NCELLS];
+    double drs_us_H2_1_plus_us_He_2[MAX_NCELLS];
+    double r_us_H2_1_plus_us_HeH_2[1024];
+    double rs_us_H2_1_plus_us_HeH_2[MAX_NCELLS];
+    double drs_us_H2_1_plus_us_HeH_2[MAX_NCELLS];
+    double r_us_H2_2_plus_de[1024];
+    double rs_us_H2_2_plus_de[MAX_NCELLS];
+    double drs_us_H2_2_plus_de[MAX_NCELLS];
+    double r_us_H2_2_plus_us_H2_1[1024];
+    double rs_us_H2_2_plus_us_H2_1[MAX_NCELLS];
+    double drs_us_H2_2_plus_us_H2_1[MAX_NCELLS];
+    double r_us_H2_2_plus_us_He_1[1024];
+    double rs_us_H2_2_plus_us_He_1[MAX_NCELLS];
+    double drs_us_H2_2_plus_us_He_1[MAX_NCELLS];
+    double r_us_H3_2_plus_de[1024];
+    double rs_us_H3_2_plus_de[MAX_NCELLS];
+    double drs_us_H3_2_plus_de[MAX_NCELLS];
+    double r_us_H_0_plus_us_H2_2[1024];
+    double rs_us_H_0_plus_us_H2_2[MAX_NCELLS];
+    double drs_us_H_0_plus_us_H2_2[MAX_NCELLS];
+    double r_us_H_0_plus_us_H3_2[1024];
+    double rs_us_H_0_plus_us_H3_2[MAX_NCELLS];
+    double drs_us_H_0_plus_us_H3_2[MAX_NCELLS];
+    double r_us_H_0_plus_us_H_1[1024];
+    double rs_us_H_0_plus_us_H_1[MAX_NCELLS];
+    double drs_us_H_0_plus_us_H_1[MAX_NCELLS];
+    double r_us_H_0_plus_us_H_2[1024];
+    double rs_us_H_0_plus_us_H_2[MAX_NCELLS];
+    double drs_us_H_0_plus_us_H_2[MAX_NCELLS];
+    double r_us_H_0_plus_us_He_2[1024];
+    double rs_us_H_0_plus_us_He_2[MAX_NCELLS];
+    double drs_us_H_0_plus_us_He_2[MAX_NCELLS];
+    double r_us_H_1_plus_us_H2_1[1024];
+    double rs_us_H_1_plus_us_H2_1[MAX_NCELLS];
+    double drs_us_H_1_plus_us_H2_1[MAX_NCELLS];
+    double r_us_H_1_plus_us_H2_2[1024];
+    double rs_us_H_1_plus_us_H2_2[MAX_NCELLS];
+    double drs_us_H_1_plus_us_H2_2[MAX_NCELLS];
+    double r_us_H_1_plus_us_He_2[1024];
+    double rs_us_H_1_plus_us_He_2[MAX_NCELLS];
+    double drs_us_H_1_plus_us_He_2[MAX_NCELLS];
+    double r_us_H_1_plus_us_HeH_2[1024];
+    double rs_us_H_1_plus_us_HeH_2[MAX_NCELLS];
+    double drs_us_H_1_plus_us_HeH_2[MAX_NCELLS];
+    double r_us_HeH_2_plus_de[1024];
+    double rs_us_HeH_2_plus_de[MAX_NCELLS];
+    double drs_us_HeH_2_plus_de[MAX_NCELLS];
+    int bin_id[MAX_NCELLS];
+    int ncells;
+    /* Device-resident tables and scratch, owned by the HIP library. */
+    void *device_state;
+} umist_data;
+
+/* Table fields in struct order, for code that walks every table. */
+/* Field names in solver order, as <name>_setup_data reports them. */
+#define UMIST_FIELD_NAMES {"de", "ge", "us_H2_1", "us_H2_2", "us_H3_2", "us_H_0", "us_H_1", "us_H_2", "us_HeH_2", "us_He_1", "us_He_2"}
+#define UMIST_TABLE_FIELDS(X) \
+    X(r_us_H2_1_plus_de) \
+    X(r_us_H2_1_plus_us_H2_1) \
+    X(r_us_H2_1_plus_us_He_2) \
+    X(r_us_H2_1_plus_us_HeH_2) \
+    X(r_us_H2_2_plus_de) \
+    X(r_us_H2_2_plus_us_H2_1) \
+    X(r_us_H2_2_plus_us_He_1) \
+    X(r_us_H3_2_plus_de) \
+    X(r_us_H_0_plus_us_H2_2) \
+    X(r_us_H_0_plus_us_H3_2) \
+    X(r_us_H_0_plus_us_H_1) \
+    X(r_us_H_0_plus_us_H_2) \
+    X(r_us_H_0_plus_us_He_2) \
+    X(r_us_H_1_plus_us_H2_1) \
+    X(r_us_H_1_plus_us_H2_2) \
+    X(r_us_H_1_plus_us_He_2) \
+    X(r_us_H_1_plus_us_HeH_2) \
+    X(r_us_HeH_2_plus_de)
+/* Values per table in struct order: the T grid, or the z grid for a redshift table. */
+#define UMIST_TABLE_LENGTHS {1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024, 1024}
+
+#include <stddef.h>
+
+#include "dengo_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* Drop-in entry points (rates_and_rate_tables.C.template:17-54, 149-274,
+   278-304, 309-350, 460-599; the reference's double &dt is double *dt). */
+umist_data *umist_setup_data(int *NumberOfFields, char ***FieldNames);
+void umist_read_rate_tables(umist_data *data);
+void umist_read_cooling_tables(umist_data *data);
+void umist_free_data(umist_data *data);
+/* rates_and_rate_tables.C.template:56-147: reads umist_initial_conditions.h5,
+   integrates to UMIST_STOP_TIME, writes umist_solution.h5 (the .bin forms of
+   dengo_amd.tables without HDF5); 0 on success, 1 on failure. */
+int umist_main(int argc, char **argv);
+double dengo_evolve_umist(double dtf, double *dt, double z, double *input,
+        double *rtol, double *atol, long long dims, umist_data *data);
+int calculate_rhs_umist(double *input, double *rhs, int nstrip, int nchem, void *sdata);
+int calculate_jacobian_umist(double *input, double *Joutput, int nstrip, int nchem, void *sdata);
+void umist_calculate_temperature(umist_data *data, double *input, int nstrip, int nchem);
+
+/* Backend extensions (no counterpart in the reference).
+   Allocate an empty struct; set tables from memory (tables[t * nT + i] in
+   struct order, T grid log-spaced on [T0, T1]); read the table blob from an
+   explicit path. */
+umist_data *umist_alloc_data(void);
+int umist_set_tables(umist_data *data, const double *tables, int ntables, int nT,
+        double T0, double T1);
+umist_data *umist_setup_data_from(const char *path, int *NumberOfFields, char ***FieldNames);
+/* sizeof(umist_data), for bindings that hold the struct as bytes. */
+size_t umist_data_bytes(void);
+/* Integrate dims cells held in device memory (AoS [dims][NSPECIES], updated
+   in place), enqueued on stream (a hipStream_t, NULL = default).  dt0 < 0
+   selects dtf/1e5.  With ttot_min non-NULL the call synchronises and returns
+   the smallest elapsed time and that cell's final dt. */
+int dengo_umist_evolve_device(umist_data *data, const dengo_options *opt, double dtf,
+        double dt0, double z, double *d_input, const double *d_rtol, double *d_atol,
+        long long dims, const dengo_cell_outputs *outputs, double *ttot_min,
+        double *dt_min, void *stream);
+/* Host-array form of dengo_evolve_umist with options and per-cell outputs. */
+double dengo_umist_evolve_host(umist_data *data, const dengo_options *opt, double dtf,
+        double *dt, double z, double *input, double *rtol, double *atol, long long dims,
+        int *status, int *nsub, int *nrej, int *nsweep, double *dt_cell, double *T_cell);
+/* Batched RHS f[dims][N], Jacobian J[dims][N][N] (column-major per cell, may
+   be NULL), temperature T[dims] and interpolated tables rates[dims][NTABLES]
+   (may be NULL) for number-density states on host memory; returns the number
+   of cells whose RHS saw a negative value (their f and J are zero), or a
+   negative error code. */
+long long dengo_umist_eval(umist_data *data, const double *u, long long dims, double z,
+        double *f, double *J, double *T, double *rates);
+/* BE_chem_solve(calculate_rhs_umist, calculate_jacobian_umist, ...) as the Cython
+   harness calls it (dengo/solvers/BE_chem_solve.C:79-241,
+   rates_and_rate_tables_run.pyx.template:142) on host arrays: one backward-Euler
+   step of nstrip cells that share dt and the Newton convergence decision; u and
+   atol are scaled and unscaled in place as the reference does; the redshift is
+   data->current_z.  Returns 0 (converged), 1 (not converged in 10 sweeps, or a
+   failed RHS/Jacobian/NaN update), or a negative error code. */
+int dengo_umist_be_chem_solve(umist_data *data, double *u, double dt, double *rtol,
+        double *atol, int nstrip, int nchem, double *scaling);
+/* ensure_electron_consistency (rates_and_rate_tables.C.template:604-630): de of
+   cell i becomes the free-electron total of cells 0..i (the reference's total
+   runs across the strip). */
+void ensure_electron_consistency_umist(double *input, int nstrip, int nchem);
+/* temperature_from_mass_density (rates_and_rate_tables.C.template:635-668): the
+   temperature of nstrip cells given as mass densities (ge in erg/g), clamped below
+   at 1 K only; computed on the GPU. */
+void temperature_from_mass_density_umist(double *input, int nstrip, int nchem,
+        double *strip_temperature);
+/* The same two helpers under the reference's unsuffixed names
+   (rates_and_rate_tables.h.template:106-107): this library holds one network. */
+void ensure_electron_consistency(double *input, int nstrip, int nchem);
+void temperature_from_mass_density(double *input, int nstrip, int nchem,
+        double *strip_temperature);
+/* Sets data->current_z, the redshift the strip solve uses. */
+void dengo_umist_set_current_z(umist_data *data, double z);
+/* Copies data->Ts (temperatures of the last strip RHS evaluation) for
+   min(nstrip, MAX_NCELLS) cells; returns that count. */
+int dengo_umist_strip_temperatures(const umist_data *data, double *T, int nstrip);
+#ifdef __cplusplus
+}
+static inline double dengo_evolve_umist(double dtf, double &dt, double z, double *input,
+        double *rtol, double *atol, long long dims, umist_data *data)
+{
+    return dengo_evolve_umist(dtf, &dt, z, input, rtol, atol, dims, data);
+}
+#endif
+
+#endif

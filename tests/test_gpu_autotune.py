@@ -1,8 +1,8 @@
 """The autotuner on the GPU: on samples of the primordial and oxygen
-workloads the accuracy bar rejects the mixed sweep for the primordial network
-(whose capped cells it does not reproduce) and accepts it for oxygen; the
-choice is the fastest accepted variant; a cached result is returned without
-re-running."""
+workloads every fast sweep meets the accuracy bar (the mixed sweep solves the
+primordial network's cells with species at the 1e-25 floor with the FP64 LU,
+so it keeps the reference's capped cells); the choice is the fastest accepted
+variant; a cached result is returned without re-running."""
 import json
 import os
 
@@ -28,9 +28,13 @@ def test_primordial_and_oxygen_choices(solver, tmp_path):
     x, rt, at = _dev(inp, *tolerances(inp))
     print("autotune primordial, 2048-cell sample", flush=True)
     r = autotune.tune(solver, x, rt, at, 1e12, sample=2048, cache=cache)
-    assert r["precision"] == "fp64-fast", r["precision_trials"]
-    assert not r["precision_trials"]["mixed"]["accepted"]
-    assert r["options"]["precision"] == runtime.FP64_FAST
+    trials = r["precision_trials"]
+    for v in ("mixed", "fp64-fast"):
+        assert trials[v]["accepted"] and trials[v]["status_agreement"] >= 0.99, trials
+    accepted = {k: v["kernel_ms"] for k, v in trials.items() if v["accepted"]}
+    assert r["precision"] == min(accepted, key=accepted.get)
+    assert r["options"]["precision"] == {"fp64": runtime.FP64, "mixed": runtime.MIXED,
+                                         "fp64-fast": runtime.FP64_FAST}[r["precision"]]
     assert r["schedule"] in r["schedule_ms"]
     # the options run as given
     y = x[:64].clone()
