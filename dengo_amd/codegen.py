@@ -54,6 +54,11 @@ def render_data_header(net, solver_name, single=False):
     w("#define %s_T_LO %r" % (solver_name.upper(), float(net.T_bounds[0])))
     w("#define %s_T_HI %r" % (solver_name.upper(), float(net.T_bounds[1])))
     w("#define %s_STOP_TIME %r" % (solver_name.upper(), float(net.stop_time)))
+    w("/* ChemicalNetwork(write_intermediate=...): dengo_evolve writes")
+    w("   <name>_intermediate_%%06d.h5 after every accepted substep")
+    w("   (rates_and_rate_tables.C.template:212-241). */")
+    w("#define %s_WRITE_INTERMEDIATE %d" % (solver_name.upper(),
+                                           1 if getattr(net, "write_intermediate_solutions", False) else 0))
     w("/* Redshift grid of the pi/ph tables (rates_and_rate_tables.C.template:31-36). */")
     w("#define %s_NZ %d" % (solver_name.upper(), nz))
     w("#define %s_Z_LO %r" % (solver_name.upper(), float(net.z_bounds[0])))
@@ -178,6 +183,16 @@ def render_data_header(net, solver_name, single=False):
         w("void ensure_electron_consistency(double *input, int nstrip, int nchem);")
         w("void temperature_from_mass_density(double *input, int nstrip, int nchem,")
         w("        double *strip_temperature);")
+    w("/* Turns the intermediate-solution files of dengo_evolve_%s and %s_main on (1)" % (n, n))
+    w("   or off (0); the default is %s_WRITE_INTERMEDIATE.  While on, the host-array" % n.upper())
+    w("   evolve runs the reference-faithful FP64 sweep and, after it, writes file k for")
+    w("   every k up to the largest accepted-substep count (at most what 1 GiB of device")
+    w("   records holds): each cell's state before its k-th accepted substep (number")
+    w("   densities), /T of its last sweep, the root attributes time and timestep of")
+    w("   the cell with the least elapsed time, and per-cell cell_time and cell_timestep")
+    w("   attributes; a cell with fewer substeps repeats its last record (NaN without")
+    w("   one).  Returns 0, or a negative error code. */")
+    w("int dengo_%s_set_write_intermediate(%s_data *data, int on);" % (n, n))
     w("/* Sets data->current_z, the redshift the strip solve uses. */")
     w("void dengo_%s_set_current_z(%s_data *data, double z);" % (n, n))
     w("/* Copies data->Ts (temperatures of the last strip RHS evaluation) for")
@@ -253,6 +268,8 @@ def render_instantiation(net, solver_name, kind=None, single=False):
           "    static constexpr double T_LO = %s_T_LO;" % N,
           "    static constexpr double T_HI = %s_T_HI;" % N,
           "    static constexpr double STOP_TIME = %s_STOP_TIME;" % N,
+          "    static constexpr bool WRITE_INTERMEDIATE = %s_WRITE_INTERMEDIATE != 0;" % N,
+          "    static constexpr const char *NAME = \"%s\";" % n,
           "    static constexpr int NZ = %s_NZ;" % N,
           "    static constexpr double Z_LO = %s_Z_LO;" % N,
           "    static constexpr double Z_HI = %s_Z_HI;" % N,

@@ -58,6 +58,10 @@ struct KParams {
     int *status, *nsub, *nrej, *nsweep;
     double *dt_out, *ttot_out, *T_out;
     unsigned long long *counter;   // persistent schedule: next cell to hand out
+    // intermediate solutions (faithful sweep only): record k of cell i at
+    // traj + (k * ncells + i) * (N + 3), for the first traj_cap accepted substeps
+    double *traj;
+    long long traj_cap;
 };
 
 template <class Net>
@@ -232,6 +236,18 @@ DENGO_D bool end_be(Cell<Net> &c, int code, const KParams<Net> &p)
         c.siter++;
         c.ttot += c.dt;
         c.dt = (c.dt * 1.1 < p.dtf - c.ttot) ? c.dt * 1.1 : p.dtf - c.ttot;
+        // what the reference's intermediate file of this substep holds
+        // (rates_and_rate_tables.C.template:212-241): the state before it (in
+        // number densities, not yet floored), the last sweep's temperature, the
+        // elapsed time and the next dt
+        if (p.traj && c.siter <= p.traj_cap) {
+            double *r = p.traj + ((c.siter - 1) * p.ncells + c.id) * (N + 3);
+#pragma unroll
+            for (int k = 0; k < N; ++k) r[Net::ext(k)] = c.prev[k];
+            r[N] = c.T;
+            r[N + 1] = c.ttot;
+            r[N + 2] = c.dt;
+        }
 #pragma unroll
         for (int i = 0; i < N; ++i) c.prev[i] = c.u[i];
 #pragma unroll

@@ -41,6 +41,8 @@ struct Api {
     herr_t (*set_attr_double)(hid_t, const char *, const char *, const double *, size_t);
     herr_t (*get_attr_double)(hid_t, const char *, const char *, double *);
     herr_t (*find_attribute)(hid_t, const char *);
+    herr_t (*get_attr_ndims)(hid_t, const char *, const char *, int *);
+    herr_t (*get_attr_info)(hid_t, const char *, const char *, hsize_t *, int *, size_t *);
 };
 
 std::once_flag g_once;
@@ -83,7 +85,9 @@ void load_api()
         sym(h, "H5LTmake_dataset_double", a.make_double) &&
         sym(h, "H5LTset_attribute_double", a.set_attr_double) &&
         sym(h, "H5LTget_attribute_double", a.get_attr_double) &&
-        sym(h, "H5LTfind_attribute", a.find_attribute);
+        sym(h, "H5LTfind_attribute", a.find_attribute) &&
+        sym(h, "H5LTget_attribute_ndims", a.get_attr_ndims) &&
+        sym(h, "H5LTget_attribute_info", a.get_attr_info);
     if (!all) {
         snprintf(g_why, sizeof(g_why), "libhdf5_hl lacks an H5LT entry point");
         return;
@@ -190,15 +194,53 @@ int dengo_h5_set_attribute(dengo_h5_file f, const char *name, double value)
                : 0;
 }
 
-int dengo_h5_get_attribute(dengo_h5_file f, const char *name, double *value)
+int dengo_h5_set_attribute_array(dengo_h5_file f, const char *name, const double *values,
+                                 long long n)
+{
+    const dengo::Api *a = dengo::api();
+    if (!a) return -1;
+    if (!values || n <= 0) return dengo::set_error("bad attribute buffer");
+    return a->set_attr_double((dengo::hid_t)f, "/", name, values, (size_t)n) < 0
+               ? dengo::set_error("cannot write attribute %s", name)
+               : 0;
+}
+
+long long dengo_h5_attribute_length(dengo_h5_file f, const char *name)
 {
     const dengo::Api *a = dengo::api();
     if (!a) return -1;
     if (a->find_attribute((dengo::hid_t)f, name) <= 0)
         return dengo::set_error("no attribute %s", name);
-    return a->get_attr_double((dengo::hid_t)f, "/", name, value) < 0
+    int rank = -1;
+    if (a->get_attr_ndims((dengo::hid_t)f, "/", name, &rank) < 0 || rank < 0 || rank > 8)
+        return dengo::set_error("cannot read the shape of attribute %s", name);
+    dengo::hsize_t dims[8] = {0};
+    int cls = 0;
+    size_t size = 0;
+    if (a->get_attr_info((dengo::hid_t)f, "/", name, dims, &cls, &size) < 0)
+        return dengo::set_error("cannot read the shape of attribute %s", name);
+    long long n = 1;
+    for (int i = 0; i < rank; ++i) n *= (long long)dims[i];
+    return n;
+}
+
+int dengo_h5_get_attribute_array(dengo_h5_file f, const char *name, double *values, long long n)
+{
+    const dengo::Api *a = dengo::api();
+    if (!a) return -1;
+    if (!values) return dengo::set_error("bad attribute buffer");
+    const long long len = dengo_h5_attribute_length(f, name);
+    if (len < 0) return -1;
+    if (len != n)
+        return dengo::set_error("attribute %s holds %lld values, not %lld", name, len, n);
+    return a->get_attr_double((dengo::hid_t)f, "/", name, values) < 0
                ? dengo::set_error("cannot read attribute %s", name)
                : 0;
+}
+
+int dengo_h5_get_attribute(dengo_h5_file f, const char *name, double *value)
+{
+    return dengo_h5_get_attribute_array(f, name, value, 1);
 }
 
 int dengo_h5_is_hdf5(const char *path)

@@ -15,6 +15,8 @@ struct Traits {
     static constexpr double T_LO = OXYGEN_T_LO;
     static constexpr double T_HI = OXYGEN_T_HI;
     static constexpr double STOP_TIME = OXYGEN_STOP_TIME;
+    static constexpr bool WRITE_INTERMEDIATE = OXYGEN_WRITE_INTERMEDIATE != 0;
+    static constexpr const char *NAME = "oxygen";
     static constexpr int NZ = OXYGEN_NZ;
     static constexpr double Z_LO = OXYGEN_Z_LO;
     static constexpr double Z_HI = OXYGEN_Z_HI;

@@ -14,6 +14,8 @@ struct Traits {
     static constexpr double T_LO = PRIMORDIAL_T_LO;
     static constexpr double T_HI = PRIMORDIAL_T_HI;
     static constexpr double STOP_TIME = PRIMORDIAL_STOP_TIME;
+    static constexpr bool WRITE_INTERMEDIATE = PRIMORDIAL_WRITE_INTERMEDIATE != 0;
+    static constexpr const char *NAME = "primordial";
     static constexpr int NZ = PRIMORDIAL_NZ;
     static constexpr double Z_LO = PRIMORDIAL_Z_LO;
     static constexpr double Z_HI = PRIMORDIAL_Z_HI;

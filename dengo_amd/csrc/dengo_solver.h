@@ -20,6 +20,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <limits>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -444,7 +445,16 @@ struct DevState {
     double *d_u0 = nullptr, *d_sc = nullptr, *d_inv = nullptr, *d_unew = nullptr;
     int *d_code = nullptr;
     OrderBufs ord;                     // cell order: rows and outputs in hand-out order
+    // intermediate solutions: -1 follows the network's WRITE_INTERMEDIATE; the
+    // records exist only while a host-array evolve that writes them runs
+    int write_im = -1;
+    double *d_traj = nullptr;
+    long long traj_cap = 0;
 };
+
+// Device memory for intermediate-solution records, (N + 3) doubles per cell
+// and accepted substep: the records stop at the substep this budget reaches.
+constexpr size_t kIntermediateBytes = size_t(1) << 30;
 
 inline int check_options(const dengo_options *opt, dengo_options *o)
 {
@@ -462,7 +472,8 @@ inline int check_options(const dengo_options *opt, dengo_options *o)
 
 // Traits (per network): Data (struct), NSPEC, NTABLES, NT, T_LO/T_HI (the
 // temperature grid), NZ, Z_LO/Z_HI (the redshift grid), STOP_TIME (<n>_main's
-// dtf), table_offsets(), table_lengths() (NT, or NZ for a redshift table),
+// dtf), WRITE_INTERMEDIATE (dengo_evolve writes <NAME>_intermediate_%06d.h5),
+// NAME, table_offsets(), table_lengths() (NT, or NZ for a redshift table),
 // table_names() (struct field names), field_names().
 template <class Net, class Tr>
 struct Host {
@@ -984,6 +995,9 @@ struct Host {
         p.ttot_out = out.ttot;
         p.T_out = out.T;
         p.counter = counter;
+        // intermediate records come from the reference-faithful sweep only
+        p.traj = o.precision == DENGO_FP64 ? s->d_traj : nullptr;
+        p.traj_cap = s->traj_cap;
         if (dims <= 0) return 0;
         void (*kp)(KParams<Net>) = nullptr;
         void (*kc)(KParams<Net>) = nullptr;
@@ -1067,6 +1081,144 @@ struct Host {
         if (dims <= 0) return 0.0;
         DevState *s = state(d);
         if (ensure_device(d, s) || ensure_scratch(s, dims)) return -1.0;
+        if (writes_intermediate(s)) {
+            const double t = evolve_host_intermediate(d, s, opt, dtf, dt, z, input, rtol, atol,
+                                                      dims, status, nsub, nrej, nsweep, dt_cell,
+                                                      T_cell);
+            if (s->d_traj) (void)hipFree(s->d_traj);
+            s->d_traj = nullptr;
+            s->traj_cap = 0;
+            return t;
+        }
+        return evolve_host_plain(d, s, opt, dtf, dt, z, input, rtol, atol, dims, status, nsub,
+                                 nrej, nsweep, dt_cell, T_cell);
+    }
+
+    static bool writes_intermediate(const DevState *s)
+    {
+        return s->write_im < 0 ? Tr::WRITE_INTERMEDIATE : s->write_im != 0;
+    }
+
+    static int set_write_intermediate(Data *d, int on)
+    {
+        if (!d) return set_error("null argument");
+        state(d)->write_im = on ? 1 : 0;
+        return 0;
+    }
+
+    // dengo_evolve with <n>_intermediate_%06d.h5 (rates_and_rate_tables.C.template:
+    // 212-241): the faithful sweep records, per cell and accepted substep, the
+    // state before it, the last sweep's T, the elapsed time and the next dt;
+    // file k then holds every cell's k-th record.  The reference's cells share
+    // their substeps; here each cell has its own, so a cell with fewer than k
+    // repeats its last record (NaN without one), the root attributes time and
+    // timestep are those of the cell with the least elapsed time (the rule of
+    // <n>_solution.h5), and cell_time / cell_timestep hold every cell's.
+    static double evolve_host_intermediate(Data *d, DevState *s, const dengo_options *opt,
+                                           double dtf, double *dt, double z, double *input,
+                                           double *rtol, double *atol, long long dims,
+                                           int *status, int *nsub, int *nrej, int *nsweep,
+                                           double *dt_cell, double *T_cell)
+    {
+        dengo_options o;
+        if (check_options(opt, &o)) return -1.0;
+        o.precision = DENGO_FP64;
+        o.order = nullptr;
+        const size_t rec = (size_t)dims * (N + 3) * sizeof(double);
+        long long cap = (long long)(kIntermediateBytes / rec);
+        if (cap > o.max_substeps) cap = o.max_substeps;
+        if (cap < 1) {
+            set_error("%lld cells do not fit one intermediate record in %zu bytes", dims,
+                      kIntermediateBytes);
+            return -1.0;
+        }
+        if (hipMalloc(&s->d_traj, (size_t)cap * rec) != hipSuccess) {
+            s->d_traj = nullptr;
+            set_error("cannot allocate %zu bytes of intermediate records", (size_t)cap * rec);
+            return -1.0;
+        }
+        s->traj_cap = cap;
+        std::vector<int> ns((size_t)dims);
+        const double ttot = evolve_host_plain(d, s, &o, dtf, dt, z, input, rtol, atol, dims, status,
+                                              ns.data(), nrej, nsweep, dt_cell, T_cell);
+        if (ttot < 0) return ttot;
+        if (nsub) memcpy(nsub, ns.data(), (size_t)dims * sizeof(int));
+        long long K = 0;
+        bool cut = false;
+        for (long long j = 0; j < dims; ++j) {
+            const long long m = ns[j] < cap ? ns[j] : cap;
+            cut = cut || ns[j] > cap;
+            if (m > K) K = m;
+        }
+        std::vector<double> recs((size_t)K * dims * (N + 3));
+        if (K > 0 &&
+            hipMemcpy(recs.data(), s->d_traj, recs.size() * sizeof(double),
+                      hipMemcpyDeviceToHost) != hipSuccess) {
+            set_error("device-to-host copy failed");
+            return -1.0;
+        }
+        if (cut)
+            fprintf(stderr, "dengo_evolve_%s: intermediate files stop at substep %lld (%zu bytes "
+                            "of records)\n", Tr::NAME, cap, kIntermediateBytes);
+        if (write_intermediate_files(recs.data(), ns.data(), cap, K, dims) != 0) return -1.0;
+        return ttot;
+    }
+
+    static int write_intermediate_files(const double *recs, const int *ns, long long cap,
+                                        long long K, long long dims)
+    {
+        const double nan = std::numeric_limits<double>::quiet_NaN();
+        std::vector<std::vector<double>> cols(N + 3, std::vector<double>((size_t)dims));
+        const char *const *fields = Tr::field_names();
+        std::vector<std::string> names(fields, fields + N);
+        names.push_back("T");
+        char path[512], name[128];
+        for (long long k = 1; k <= K; ++k) {
+            double tmin = nan, dtmin = nan;
+            for (long long j = 0; j < dims; ++j) {
+                const long long m = ns[j] < cap ? ns[j] : cap;
+                if (m == 0) {
+                    for (int i = 0; i < N + 3; ++i) cols[i][j] = nan;
+                    continue;
+                }
+                const double *r = recs + ((size_t)((k <= m ? k : m) - 1) * dims + j) * (N + 3);
+                for (int i = 0; i < N + 3; ++i) cols[i][j] = r[i];
+                if (!(r[N + 1] >= tmin)) {
+                    tmin = r[N + 1];
+                    dtmin = r[N + 2];
+                }
+            }
+            int rc = 0;
+            if (dengo_hdf5_available()) {
+                snprintf(path, sizeof(path), "%s_intermediate_%06lld.h5", Tr::NAME, k);
+                const dengo_h5_file f = dengo_h5_open(path, 1);
+                if (f < 0) return -1;
+                for (int i = 0; rc == 0 && i <= N; ++i) {
+                    snprintf(name, sizeof(name), "/%s", names[i].c_str());
+                    rc = dengo_h5_write(f, name, cols[i].data(), dims);
+                }
+                if (rc == 0) rc = dengo_h5_set_attribute(f, "time", tmin);
+                if (rc == 0) rc = dengo_h5_set_attribute(f, "timestep", dtmin);
+                if (rc == 0) rc = dengo_h5_set_attribute_array(f, "cell_time", cols[N + 1].data(), dims);
+                if (rc == 0)
+                    rc = dengo_h5_set_attribute_array(f, "cell_timestep", cols[N + 2].data(), dims);
+                if (dengo_h5_close(f) != 0) rc = -1;
+            } else {
+                snprintf(path, sizeof(path), "%s_intermediate_%06lld.bin", Tr::NAME, k);
+                std::vector<const double *> p;
+                for (int i = 0; i <= N; ++i) p.push_back(cols[i].data());
+                rc = write_fields_bin(path, names, p, dims, tmin, dtmin);
+            }
+            if (rc != 0) return -1;
+        }
+        return 0;
+    }
+
+    static double evolve_host_plain(Data *d, DevState *s, const dengo_options *opt, double dtf,
+                                    double *dt, double z, double *input, double *rtol,
+                                    double *atol, long long dims, int *status, int *nsub,
+                                    int *nrej, int *nsweep, double *dt_cell, double *T_cell)
+    {
         const size_t bytes = (size_t)dims * N * sizeof(double);
         if (hipMemcpy(s->d_in, input, bytes, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(s->d_rtol, rtol, bytes, hipMemcpyHostToDevice) != hipSuccess ||
@@ -1460,6 +1612,10 @@ struct Host {
                                    int *st, int *ns, int *nr, int *nw, double *dtc, double *Tc) \
     {                                                                                          \
         return H::evolve_host(d, o, dtf, dt, z, x, rt, at, dims, st, ns, nr, nw, dtc, Tc);     \
+    }                                                                                          \
+    int dengo_##n##_set_write_intermediate(n##_data *d, int on)                                \
+    {                                                                                          \
+        return H::set_write_intermediate(d, on);                                               \
     }                                                                                          \
     double dengo_evolve_##n(double dtf, double *dt, double z, double *input, double *rtol,     \
                             double *atol, long long dims, n##_data *data)                      \

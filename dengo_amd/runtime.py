@@ -106,6 +106,7 @@ def _network_signatures(n):
         "ensure_electron_consistency_%s" % n: (None, [_dp, ctypes.c_int, ctypes.c_int]),
         "temperature_from_mass_density_%s" % n: (None, [_dp, ctypes.c_int, ctypes.c_int, _dp]),
         "dengo_%s_set_current_z" % n: (None, [ctypes.c_void_p, ctypes.c_double]),
+        "dengo_%s_set_write_intermediate" % n: (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
         "dengo_%s_strip_temperatures" % n: (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int]),
         "%s_main" % n: (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p]),
         "%s_data_bytes" % n: (ctypes.c_size_t, []),
@@ -126,6 +127,11 @@ COMMON_SIGNATURES = {
     "dengo_h5_set_attribute": (ctypes.c_int, [ctypes.c_longlong, ctypes.c_char_p,
                                               ctypes.c_double]),
     "dengo_h5_get_attribute": (ctypes.c_int, [ctypes.c_longlong, ctypes.c_char_p, _dp]),
+    "dengo_h5_set_attribute_array": (ctypes.c_int, [ctypes.c_longlong, ctypes.c_char_p, _dp,
+                                                    ctypes.c_longlong]),
+    "dengo_h5_attribute_length": (ctypes.c_longlong, [ctypes.c_longlong, ctypes.c_char_p]),
+    "dengo_h5_get_attribute_array": (ctypes.c_int, [ctypes.c_longlong, ctypes.c_char_p, _dp,
+                                                    ctypes.c_longlong]),
     "dengo_h5_is_hdf5": (ctypes.c_int, [ctypes.c_char_p]),
 }
 
@@ -409,6 +415,13 @@ class NetworkSolver(object):
         T = np.full(u.shape[0], np.nan)
         self._fn("temperature_from_mass_density_%s")(_p(u), u.shape[0], u.shape[1], _p(T))
         return T
+
+    def set_write_intermediate(self, on):
+        """Turns the <name>_intermediate_%06d.h5 files of the host-array evolve
+        (written to the working directory) on or off."""
+        if self._fn("dengo_%s_set_write_intermediate")(self.data, 1 if on else 0) != 0:
+            raise RuntimeError("dengo_%s_set_write_intermediate: %s" % (self.name,
+                                                                         last_error(self.lib)))
 
     def set_current_z(self, z):
         self._fn("dengo_%s_set_current_z")(self.data, z)

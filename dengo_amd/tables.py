@@ -115,7 +115,7 @@ def _check(rc, what):
 
 def write_h5(path, datasets, attrs=None):
     """Creates an HDF5 file of 1-D FP64 datasets {name: array} (names without
-    a leading '/') and scalar FP64 root attributes {name: value}."""
+    a leading '/') and FP64 root attributes {name: value or 1-D array}."""
     lib = _h5()
     f = _check(lib.dengo_h5_open(path.encode(), 1), "dengo_h5_open")
     try:
@@ -125,14 +125,21 @@ def write_h5(path, datasets, attrs=None):
                                       a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), a.size),
                    "dengo_h5_write %s" % k)
         for k, v in (attrs or {}).items():
-            _check(lib.dengo_h5_set_attribute(f, k.encode(), float(v)), "attribute %s" % k)
+            if np.ndim(v) == 0:
+                _check(lib.dengo_h5_set_attribute(f, k.encode(), float(v)), "attribute %s" % k)
+            else:
+                a = np.ascontiguousarray(v, dtype=np.float64).ravel()
+                _check(lib.dengo_h5_set_attribute_array(
+                    f, k.encode(), a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), a.size),
+                    "attribute %s" % k)
     finally:
         lib.dengo_h5_close(f)
 
 
 def read_h5(path, names, attrs=()):
     """({name: array}, {attr: value}) of the named datasets and root
-    attributes of an HDF5 file."""
+    attributes of an HDF5 file (an attribute of more than one value comes back
+    as an array)."""
     lib = _h5()
     f = _check(lib.dengo_h5_open(path.encode(), 0), "dengo_h5_open")
     out, at = {}, {}
@@ -145,9 +152,12 @@ def read_h5(path, names, attrs=()):
                    "dengo_h5_read %s" % k)
             out[k] = a
         for k in attrs:
-            v = ctypes.c_double()
-            _check(lib.dengo_h5_get_attribute(f, k.encode(), ctypes.byref(v)), "attribute %s" % k)
-            at[k] = v.value
+            n = _check(lib.dengo_h5_attribute_length(f, k.encode()), "attribute %s" % k)
+            v = np.empty(n, dtype=np.float64)
+            _check(lib.dengo_h5_get_attribute_array(
+                f, k.encode(), v.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n),
+                "attribute %s" % k)
+            at[k] = float(v[0]) if n == 1 else v
     finally:
         lib.dengo_h5_close(f)
     return out, at

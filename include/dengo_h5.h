@@ -44,6 +44,13 @@ int dengo_h5_write(dengo_h5_file f, const char *dataset, const double *data, lon
 /* A scalar FP64 attribute of the root group (<n>_solution.h5's time, timestep). */
 int dengo_h5_set_attribute(dengo_h5_file f, const char *name, double value);
 int dengo_h5_get_attribute(dengo_h5_file f, const char *name, double *value);
+/* FP64 root attributes of n values (an extension: the per-cell times of the
+   intermediate-solution files), the number of values an attribute holds, and
+   reading exactly n of them. */
+int dengo_h5_set_attribute_array(dengo_h5_file f, const char *name, const double *values,
+                                 long long n);
+long long dengo_h5_attribute_length(dengo_h5_file f, const char *name);
+int dengo_h5_get_attribute_array(dengo_h5_file f, const char *name, double *values, long long n);
 
 /* 1 when the file starts with the HDF5 signature, 0 otherwise. */
 int dengo_h5_is_hdf5(const char *path);

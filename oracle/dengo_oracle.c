@@ -961,9 +961,23 @@ typedef struct { int status, nsub, nrej; double dt; } cell_stats;
 static int g_max_substeps = 50000;
 void orc_set_max_substeps(int n) { g_max_substeps = n > 0 ? n : 50000; }
 
+/* Intermediate-solution records (rates_and_rate_tables.C.template:212-241):
+ * when set, record k of cell c (k = 0 for its first accepted substep) is
+ * g_traj[(k * ncells + c) * (N + 3) + ...]: the state before that substep in
+ * the network's unknown order (number densities, not yet floored), the last
+ * sweep's temperature, the elapsed time and the next dt; at most g_traj_cap
+ * records per cell. */
+static double *g_traj = NULL;
+static long long g_traj_cap = 0;
+void orc_set_trajectory(double *buf, long long cap)
+{
+    g_traj = buf;
+    g_traj_cap = buf ? cap : 0;
+}
+
 static double evolve_cell(orc_be_f be, orc_prim_data *d, double dtf, double dt, double z,
                           double *in, double *rtol, double *atol, int input_is_number,
-                          cell_stats *st)
+                          cell_stats *st, long long cell, long long ncells)
 {
     const int N = d->n;
     const int ge = N - 1, de = d->gen ? d->gen->de : (d->net ? d->net->de : DE);
@@ -1003,6 +1017,13 @@ static double evolve_cell(orc_be_f be, orc_prim_data *d, double dtf, double dt, 
             siter++;
             ttot += dt;
             dt = (dt * 1.1 < dtf - ttot) ? dt * 1.1 : dtf - ttot;
+            if (g_traj && siter <= g_traj_cap) {
+                double *r = g_traj + ((siter - 1) * ncells + cell) * (N + 3);
+                for (int k = 0; k < N; ++k) r[k] = prev[k];
+                r[N] = d->T;
+                r[N + 1] = ttot;
+                r[N + 2] = dt;
+            }
             for (int k = 0; k < N; ++k) prev[k] = in[k];
             for (int k = 0; k < N; ++k)
                 if (in[k] < 1e-25) in[k] = 1e-25;
@@ -1056,7 +1077,7 @@ static double evolve_all(const orc_ma_net *net, const orc_gen_net *gen, double d
             d.ntab = gen->ntab;
         }
         double t = evolve_cell(be, &d, dtf, dt_in, z, input + c * N, rtol + c * N,
-                               atol + c * N, input_is_number, &st);
+                               atol + c * N, input_is_number, &st, c, dims);
         if (status) status[c] = st.status;
         if (nsub) nsub[c] = st.nsub;
         if (nrej) nrej[c] = st.nrej;

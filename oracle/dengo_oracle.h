@@ -101,6 +101,11 @@ typedef struct orc_prim_data {
 
 /* Accepted-substep cap (default 50000, as the reference). */
 void orc_set_max_substeps(int n);
+/* Intermediate-solution records of the per-cell evolve (rates_and_rate_tables.C.template:
+ * 212-241): record k of cell c at buf[(k * ncells + c) * (N + 3)]: the state before accepted
+ * substep k + 1 (unknown order, number densities), the last sweep's T, the elapsed time and
+ * the next dt; at most cap per cell.  NULL turns recording off. */
+void orc_set_trajectory(double *buf, long long cap);
 
 /* Take ln T with orc_plog instead of libm's log. */
 #define ORC_FLAG_PORTABLE_LOG 1

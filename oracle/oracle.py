@@ -3,6 +3,7 @@
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
 bench.py's CPU-baseline leg to check (never to produce) the GPU results.
 """
+import contextlib
 import ctypes
 import os
 
@@ -94,6 +95,25 @@ def interp_logT(tables, logT, T_bounds=(1.0, 1e9)):
     lib.orc_interp_logT(_ptr(tab), tab.shape[0], tab.shape[1], T_bounds[0], T_bounds[1],
                         _ptr(lt), lt.size, _ptr(out))
     return out
+
+
+@contextlib.contextmanager
+def trajectory(ncells, n, cap):
+    """While active, the per-cell evolve functions record the reference's
+    intermediate solutions (rates_and_rate_tables.C.template:212-241) into the
+    yielded array [cap, ncells, n + 3]: record k of a cell is the state before
+    its accepted substep k + 1 (number densities, the network's unknown order),
+    the last sweep's temperature, the elapsed time and the next dt; NaN where a
+    cell has no such substep."""
+    lib = load()
+    lib.orc_set_trajectory.restype = None
+    lib.orc_set_trajectory.argtypes = [ctypes.c_void_p, ctypes.c_longlong]
+    buf = np.full((cap, ncells, n + 3), np.nan)
+    lib.orc_set_trajectory(buf.ctypes.data, cap)
+    try:
+        yield buf
+    finally:
+        lib.orc_set_trajectory(None, 0)
 
 
 def prim_eval(u, tables, T_bounds=(1.0, 1e9), z=-1.0, flags=0):

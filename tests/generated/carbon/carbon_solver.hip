@@ -15,6 +15,8 @@ struct Traits {
     static constexpr double T_LO = CARBON_T_LO;
     static constexpr double T_HI = CARBON_T_HI;
     static constexpr double STOP_TIME = CARBON_STOP_TIME;
+    static constexpr bool WRITE_INTERMEDIATE = CARBON_WRITE_INTERMEDIATE != 0;
+    static constexpr const char *NAME = "carbon";
     static constexpr int NZ = CARBON_NZ;
     static constexpr double Z_LO = CARBON_Z_LO;
     static constexpr double Z_HI = CARBON_Z_HI;

@@ -15,6 +15,10 @@
 #define OXYDATA_T_LO 100.0
 #define OXYDATA_T_HI 1000000000.0
 #define OXYDATA_STOP_TIME 1000000000000.0
+/* ChemicalNetwork(write_intermediate=...): dengo_evolve writes
+   <name>_intermediate_%%06d.h5 after every accepted substep
+   (rates_and_rate_tables.C.template:212-241). */
+#define OXYDATA_WRITE_INTERMEDIATE 0
 /* Redshift grid of the pi/ph tables (rates_and_rate_tables.C.template:31-36). */
 #define OXYDATA_NZ 100
 #define OXYDATA_Z_LO 0.0
@@ -290,6 +294,16 @@ void temperature_from_mass_density_oxydata(double *input, int nstrip, int nchem,
 void ensure_electron_consistency(double *input, int nstrip, int nchem);
 void temperature_from_mass_density(double *input, int nstrip, int nchem,
         double *strip_temperature);
+/* Turns the intermediate-solution files of dengo_evolve_oxydata and oxydata_main on (1)
+   or off (0); the default is OXYDATA_WRITE_INTERMEDIATE.  While on, the host-array
+   evolve runs the reference-faithful FP64 sweep and, after it, writes file k for
+   every k up to the largest accepted-substep count (at most what 1 GiB of device
+   records holds): each cell's state before its k-th accepted substep (number
+   densities), /T of its last sweep, the root attributes time and timestep of
+   the cell with the least elapsed time, and per-cell cell_time and cell_timestep
+   attributes; a cell with fewer substeps repeats its last record (NaN without
+   one).  Returns 0, or a negative error code. */
+int dengo_oxydata_set_write_intermediate(oxydata_data *data, int on);
 /* Sets data->current_z, the redshift the strip solve uses. */
 void dengo_oxydata_set_current_z(oxydata_data *data, double z);
 /* Copies data->Ts (temperatures of the last strip RHS evaluation) for

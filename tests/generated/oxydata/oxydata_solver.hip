@@ -15,6 +15,8 @@ struct Traits {
     static constexpr double T_LO = OXYDATA_T_LO;
     static constexpr double T_HI = OXYDATA_T_HI;
     static constexpr double STOP_TIME = OXYDATA_STOP_TIME;
+    static constexpr bool WRITE_INTERMEDIATE = OXYDATA_WRITE_INTERMEDIATE != 0;
+    static constexpr const char *NAME = "oxydata";
     static constexpr int NZ = OXYDATA_NZ;
     static constexpr double Z_LO = OXYDATA_Z_LO;
     static constexpr double Z_HI = OXYDATA_Z_HI;

@@ -15,6 +15,10 @@
 #define PRIMGEN_T_LO 1.0
 #define PRIMGEN_T_HI 1000000000.0
 #define PRIMGEN_STOP_TIME 1000000000000.0
+/* ChemicalNetwork(write_intermediate=...): dengo_evolve writes
+   <name>_intermediate_%%06d.h5 after every accepted substep
+   (rates_and_rate_tables.C.template:212-241). */
+#define PRIMGEN_WRITE_INTERMEDIATE 0
 /* Redshift grid of the pi/ph tables (rates_and_rate_tables.C.template:31-36). */
 #define PRIMGEN_NZ 1
 #define PRIMGEN_Z_LO 0.0
@@ -306,6 +310,16 @@ void temperature_from_mass_density_primgen(double *input, int nstrip, int nchem,
 void ensure_electron_consistency(double *input, int nstrip, int nchem);
 void temperature_from_mass_density(double *input, int nstrip, int nchem,
         double *strip_temperature);
+/* Turns the intermediate-solution files of dengo_evolve_primgen and primgen_main on (1)
+   or off (0); the default is PRIMGEN_WRITE_INTERMEDIATE.  While on, the host-array
+   evolve runs the reference-faithful FP64 sweep and, after it, writes file k for
+   every k up to the largest accepted-substep count (at most what 1 GiB of device
+   records holds): each cell's state before its k-th accepted substep (number
+   densities), /T of its last sweep, the root attributes time and timestep of
+   the cell with the least elapsed time, and per-cell cell_time and cell_timestep
+   attributes; a cell with fewer substeps repeats its last record (NaN without
+   one).  Returns 0, or a negative error code. */
+int dengo_primgen_set_write_intermediate(primgen_data *data, int on);
 /* Sets data->current_z, the redshift the strip solve uses. */
 void dengo_primgen_set_current_z(primgen_data *data, double z);
 /* Copies data->Ts (temperatures of the last strip RHS evaluation) for

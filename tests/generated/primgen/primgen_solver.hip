@@ -15,6 +15,8 @@ struct Traits {
     static constexpr double T_LO = PRIMGEN_T_LO;
     static constexpr double T_HI = PRIMGEN_T_HI;
     static constexpr double STOP_TIME = PRIMGEN_STOP_TIME;
+    static constexpr bool WRITE_INTERMEDIATE = PRIMGEN_WRITE_INTERMEDIATE != 0;
+    static constexpr const char *NAME = "primgen";
     static constexpr int NZ = PRIMGEN_NZ;
     static constexpr double Z_LO = PRIMGEN_Z_LO;
     static constexpr double Z_HI = PRIMGEN_Z_HI;

@@ -15,6 +15,8 @@ struct Traits {
     static constexpr double T_LO = PRIMPHOTO_T_LO;
     static constexpr double T_HI = PRIMPHOTO_T_HI;
     static constexpr double STOP_TIME = PRIMPHOTO_STOP_TIME;
+    static constexpr bool WRITE_INTERMEDIATE = PRIMPHOTO_WRITE_INTERMEDIATE != 0;
+    static constexpr const char *NAME = "primphoto";
     static constexpr int NZ = PRIMPHOTO_NZ;
     static constexpr double Z_LO = PRIMPHOTO_Z_LO;
     static constexpr double Z_HI = PRIMPHOTO_Z_HI;

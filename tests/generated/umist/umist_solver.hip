@@ -15,6 +15,8 @@ struct Traits {
     static constexpr double T_LO = UMIST_T_LO;
     static constexpr double T_HI = UMIST_T_HI;
     static constexpr double STOP_TIME = UMIST_STOP_TIME;
+    static constexpr bool WRITE_INTERMEDIATE = UMIST_WRITE_INTERMEDIATE != 0;
+    static constexpr const char *NAME = "umist";
     static constexpr int NZ = UMIST_NZ;
     static constexpr double Z_LO = UMIST_Z_LO;
     static constexpr double Z_HI = UMIST_Z_HI;

@@ -28,6 +28,29 @@ def test_datasets_and_attributes_round_trip(tmp_path):
     assert at == {"time": 1.25e12, "timestep": 7.0}
 
 
+def test_array_attributes_round_trip(tmp_path):
+    """Per-cell root attributes (the intermediate files' cell_time and
+    cell_timestep) beside scalar ones; reading checks the stored length."""
+    p = str(tmp_path / "x.h5")
+    ct = np.array([1.0, 2.5e10, np.nan, 1e12])
+    tables.write_h5(p, {"a": np.arange(4.0)}, attrs={"time": 1.0, "cell_time": ct})
+    _, at = tables.read_h5(p, [], attrs=["time", "cell_time"])
+    assert at["time"] == 1.0
+    assert np.array_equal(at["cell_time"], ct, equal_nan=True)
+    lib = runtime.load()
+    f = lib.dengo_h5_open(p.encode(), 0)
+    try:
+        assert lib.dengo_h5_attribute_length(f, b"cell_time") == 4
+        assert lib.dengo_h5_attribute_length(f, b"time") == 1
+        v = ctypes.c_double()
+        # a scalar read of a 4-value attribute is refused, not overrun
+        assert lib.dengo_h5_get_attribute(f, b"cell_time", ctypes.byref(v)) < 0
+        assert "holds 4 values, not 1" in runtime.last_error()
+        assert lib.dengo_h5_attribute_length(f, b"nope") < 0
+    finally:
+        lib.dengo_h5_close(f)
+
+
 def test_missing_dataset_and_wrong_length_are_errors(tmp_path):
     p = str(tmp_path / "x.h5")
     tables.write_h5(p, {"a": np.arange(4.0)})
