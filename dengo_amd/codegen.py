@@ -193,6 +193,9 @@ def render_data_header(net, solver_name, single=False):
     w("   attributes; a cell with fewer substeps repeats its last record (NaN without")
     w("   one).  Returns 0, or a negative error code. */")
     w("int dengo_%s_set_write_intermediate(%s_data *data, int on);" % (n, n))
+    w("/* This network's hooks for dengo_server_add_network (include/dengo_server.h):")
+    w("   a patch server in the same process can then serve it. */")
+    w("const dengo_net_hooks *dengo_%s_server_hooks(void);" % n)
     w("/* Sets data->current_z, the redshift the strip solve uses. */")
     w("void dengo_%s_set_current_z(%s_data *data, double z);" % (n, n))
     w("/* Copies data->Ts (temperatures of the last strip RHS evaluation) for")

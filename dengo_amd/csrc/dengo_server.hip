@@ -29,11 +29,15 @@ using namespace dengo;
 
 namespace {
 
-constexpr int kNets = 3;
+// The three built-in networks take ids 0-2 (enum dengo_network); networks
+// added with dengo_server_add_network take the next ids.
+constexpr int kMaxNets = 16;
+constexpr int kBuiltinNets = 3;
 
 struct Request {
     long long ticket = -1;
     int net = 0;
+    bool device = false;   // the arrays are the caller's device memory: no copies
     long long n = 0;
     double *x = nullptr;
     const double *rtol = nullptr;
@@ -44,7 +48,26 @@ struct Request {
     double t_submit = 0;
     size_t bytes = 0;
     int bypassed = 0;
+    std::vector<long long> order;   // dense network, host submits: densest first
 };
+
+// Densest cells first: indices by descending total of the n x ns species
+// columns, a NaN total counting as -inf, ties in index order
+// (runtime.density_order follows the same rule).
+std::vector<long long> density_order(const double *x, long long n, int ns)
+{
+    std::vector<double> key((size_t)n);
+    std::vector<long long> ord((size_t)n);
+    for (long long i = 0; i < n; ++i) {
+        double t = 0.0;
+        for (int j = 0; j < ns; ++j) t += x[(size_t)i * ns + j];
+        key[(size_t)i] = std::isnan(t) ? -HUGE_VAL : t;
+        ord[(size_t)i] = i;
+    }
+    std::stable_sort(ord.begin(), ord.end(),
+                     [&key](long long a, long long b) { return key[(size_t)a] > key[(size_t)b]; });
+    return ord;
+}
 
 enum SlotPhase { kFree = 0, kRunning = 1, kCopyBack = 2 };
 
@@ -59,12 +82,59 @@ struct Slot {
     int *dst = nullptr;
     long long *dord = nullptr;         // dense network: the hand-out order
     std::vector<long long> hord;       // its host copy, alive until the slot completes
-    std::vector<double> hkey;
+    int *dcounts = nullptr;            // device submits: status counts, device and
+    int *hcounts = nullptr;            // pinned host copies
     double t_admit = 0;
     int inflight_at_admit = 0, max_inflight = 0;
 };
 
 size_t round_up(size_t n) { return Arena::round_up(n); }
+
+// Cells of a device-submitted patch by final status (0, 1, 2); one workgroup.
+template <int>
+__global__ void __launch_bounds__(256) k_status_counts(const int *status, long long n, int *out)
+{
+    __shared__ int c[3][256];
+    int a = 0, b = 0, d = 0;
+    for (long long i = threadIdx.x; i < n; i += 256) {
+        const int v = status[i];
+        a += v == 0;
+        b += v == 1;
+        d += v == 2;
+    }
+    c[0][threadIdx.x] = a;
+    c[1][threadIdx.x] = b;
+    c[2][threadIdx.x] = d;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 3; ++k) c[k][threadIdx.x] += c[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) out[threadIdx.x] = c[threadIdx.x][0];
+}
+
+// A device pointer the caller hands over must be device memory of at least
+// `bytes` from ptr on; anything else would fault the kernel.
+int check_device_range(const void *ptr, size_t bytes, const char *what)
+{
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, ptr) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error("%s is not device memory", what);
+    }
+    if (at.type != hipMemoryTypeDevice)
+        return set_error("%s is not device memory", what);
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)ptr) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error("%s: its allocation is unknown", what);
+    }
+    if ((const char *)ptr + bytes > (const char *)base + size)
+        return set_error("%s holds fewer than %zu bytes", what, bytes);
+    return 0;
+}
 
 }  // namespace
 
@@ -72,9 +142,11 @@ struct dengo_server {
     int device = 0;
     dengo_server_config cfg;
     int precision = DENGO_FP64;
-    void *data[kNets] = {nullptr, nullptr, nullptr};
-    NetHooks hooks[kNets];
-    int grid_cap[kNets] = {0, 0, 0};
+    void *data[kMaxNets] = {};
+    NetHooks hooks[kMaxNets];
+    int grid_cap[kMaxNets] = {};
+    int prec[kMaxNets];                // per network: precision, or -1 for `precision`
+    int nnets = kBuiltinNets;
     char *arena_base = nullptr;
     Arena arena;
     std::vector<Slot> slots;
@@ -82,6 +154,7 @@ struct dengo_server {
     hipEvent_t epoch = nullptr;
 
     std::mutex mu;
+    std::mutex add_mu;                 // serialises dengo_server_add_network
     std::condition_variable cv_work, cv_done;
     std::deque<Request> waiting;
     std::deque<dengo_patch_result> completed;
@@ -99,18 +172,34 @@ struct dengo_server {
 
     int net_precision(int net) const
     {
-        const int pr = cfg.precision[net] >= 0 ? cfg.precision[net] : precision;
+        const int pr = prec[net] >= 0 ? prec[net] : precision;
         if (hooks[net].isothermal && pr == DENGO_MIXED) return DENGO_FP64_FAST;
         return pr;
     }
 
-    size_t request_bytes(int net, long long n) const
+    size_t request_bytes(int net, long long n, bool device) const
     {
+        if (device) return 0;   // integrated in the caller's buffers
         const size_t row = (size_t)n * hooks[net].nspecies * sizeof(double);
         size_t b = 3 * round_up(row) + round_up((size_t)n * sizeof(int));
         // the dense network's patches also carry T[n] and the order[n]
         if (hooks[net].isothermal) b += 2 * round_up((size_t)n * sizeof(double));
         return b;
+    }
+
+    // Makes network net's tables resident and caps its persistent grid at
+    // 1/share of the grid that fills the GPU (share 0: cfg.grid_share).
+    int prepare_net(int net, int share)
+    {
+        dengo_options o;
+        dengo_default_options(&o);
+        o.precision = net_precision(net);
+        o.block = 0;
+        int full = 0;
+        if (hooks[net].prepare(data[net], &o, &full)) return -1;
+        if (share <= 0) share = cfg.grid_share;
+        grid_cap[net] = full / share > 0 ? full / share : 1;
+        return 0;
     }
 
     // Called with mu held; the slot's device work has finished (or failed).
@@ -137,11 +226,15 @@ struct dengo_server {
                 r.dev_start_ms = a;
                 r.dev_end_ms = b;
             }
-            if (s.req.status)
+            if (s.req.device) {
+                if (s.req.status)
+                    for (int v = 0; v < 3; ++v) r.status_counts[v] = s.hcounts[v];
+            } else if (s.req.status) {
                 for (long long i = 0; i < s.req.n; ++i) {
                     const int v = s.req.status[i];
                     if (v >= 0 && v < 3) r.status_counts[v]++;
                 }
+            }
             ncompleted++;
             cells_completed += s.req.n;
         } else {
@@ -162,30 +255,24 @@ struct dengo_server {
     {
         const Request &q = s.req;
         const size_t row = (size_t)q.n * hooks[q.net].nspecies * sizeof(double);
+        if (q.device) return enqueue_kernel(s);
         DENGO_HIP_OK(hipMemcpyAsync(s.dx, q.x, row, hipMemcpyHostToDevice, s.stream));
         DENGO_HIP_OK(hipMemcpyAsync(s.drt, q.rtol, row, hipMemcpyHostToDevice, s.stream));
         DENGO_HIP_OK(hipMemcpyAsync(s.dat, q.atol, row, hipMemcpyHostToDevice, s.stream));
         if (s.dT)
             DENGO_HIP_OK(hipMemcpyAsync(s.dT, q.T, (size_t)q.n * sizeof(double),
                                         hipMemcpyHostToDevice, s.stream));
-        if (s.dord) {
-            // densest cells first (total density of the species columns; a
-            // NaN total sorts last), ties in index order
-            const int ns = hooks[q.net].nspecies;
-            s.hord.resize((size_t)q.n);
-            s.hkey.resize((size_t)q.n);
-            for (long long i = 0; i < q.n; ++i) {
-                double t = 0.0;
-                for (int j = 0; j < ns; ++j) t += q.x[(size_t)i * ns + j];
-                s.hkey[(size_t)i] = std::isnan(t) ? -HUGE_VAL : t;
-                s.hord[(size_t)i] = i;
-            }
-            std::stable_sort(s.hord.begin(), s.hord.end(), [&s](long long a, long long b) {
-                return s.hkey[(size_t)a] > s.hkey[(size_t)b];
-            });
+        if (s.dord)   // the order was built by the submitting thread
             DENGO_HIP_OK(hipMemcpyAsync(s.dord, s.hord.data(), (size_t)q.n * sizeof(long long),
                                         hipMemcpyHostToDevice, s.stream));
-        }
+        return enqueue_kernel(s);
+    }
+
+    // The integration itself, on the slot's buffers, bracketed by the
+    // slot's start/end events.
+    int enqueue_kernel(Slot &s)
+    {
+        const Request &q = s.req;
         DENGO_HIP_OK(hipEventRecord(s.ev_start, s.stream));
         LaunchArgs a;
         memset(&a, 0, sizeof(a));
@@ -214,6 +301,18 @@ struct dengo_server {
     int enqueue_copy_back(Slot &s)
     {
         const Request &q = s.req;
+        if (q.device) {
+            // the state stays where it is; only the status counts come back
+            if (q.status) {
+                hipLaunchKernelGGL(k_status_counts<0>, dim3(1), dim3(256), 0, s.stream, q.status,
+                                   q.n, s.dcounts);
+                DENGO_HIP_OK(hipGetLastError());
+                DENGO_HIP_OK(hipMemcpyAsync(s.hcounts, s.dcounts, 3 * sizeof(int),
+                                            hipMemcpyDeviceToHost, s.stream));
+            }
+            DENGO_HIP_OK(hipEventRecord(s.ev_done, s.stream));
+            return 0;
+        }
         const size_t row = (size_t)q.n * hooks[q.net].nspecies * sizeof(double);
         DENGO_HIP_OK(hipMemcpyAsync(q.x, s.dx, row, hipMemcpyDeviceToHost, s.stream));
         DENGO_HIP_OK(hipMemcpyAsync(q.atol, s.dat, row, hipMemcpyDeviceToHost, s.stream));
@@ -277,24 +376,39 @@ struct dengo_server {
                     }
                 if (!fs) break;
                 const int idx = pick_admission(
-                    waiting, [&](const Request &r) { return arena.fits(r.bytes); },
+                    waiting, [&](const Request &r) { return r.device || arena.fits(r.bytes); },
                     cfg.max_bypass);
                 if (idx < 0) break;
                 Slot &s = *fs;
                 s.req = waiting[idx];
                 waiting.erase(waiting.begin() + idx);
-                s.off = arena.alloc(s.req.bytes);
-                char *p = arena_base + s.off;
-                const size_t row = round_up((size_t)s.req.n * hooks[s.req.net].nspecies * 8);
-                s.dx = (double *)p;
-                s.drt = (double *)(p + row);
-                s.dat = (double *)(p + 2 * row);
-                s.dst = (int *)(p + 3 * row);
-                const size_t st_end = 3 * row + round_up((size_t)s.req.n * sizeof(int));
-                s.dT = hooks[s.req.net].isothermal ? (double *)(p + st_end) : nullptr;
-                s.dord = hooks[s.req.net].isothermal && cfg.dense_order
-                             ? (long long *)(p + st_end + round_up((size_t)s.req.n * sizeof(double)))
-                             : nullptr;
+                if (s.req.device) {
+                    // the caller's buffers; the dense network's cells keep
+                    // index order (the densest-first order is built from host
+                    // state)
+                    s.off = -1;
+                    s.dx = s.req.x;
+                    s.drt = const_cast<double *>(s.req.rtol);
+                    s.dat = s.req.atol;
+                    s.dst = s.req.status;
+                    s.dT = const_cast<double *>(s.req.T);
+                    s.dord = nullptr;
+                } else {
+                    s.off = arena.alloc(s.req.bytes);
+                    char *p = arena_base + s.off;
+                    const size_t row = round_up((size_t)s.req.n * hooks[s.req.net].nspecies * 8);
+                    s.dx = (double *)p;
+                    s.drt = (double *)(p + row);
+                    s.dat = (double *)(p + 2 * row);
+                    s.dst = (int *)(p + 3 * row);
+                    const size_t st_end = 3 * row + round_up((size_t)s.req.n * sizeof(int));
+                    s.dT = hooks[s.req.net].isothermal ? (double *)(p + st_end) : nullptr;
+                    s.dord = hooks[s.req.net].isothermal && cfg.dense_order
+                                 ? (long long *)(p + st_end +
+                                                 round_up((size_t)s.req.n * sizeof(double)))
+                                 : nullptr;
+                    s.hord = std::move(s.req.order);
+                }
                 s.phase = kRunning;
                 s.t_admit = now();
                 inflight++;
@@ -353,23 +467,20 @@ dengo_server *dengo_server_create(const dengo_server_config *cfg_in, void *primo
         set_error("unknown precision %d", precision);
         return nullptr;
     }
-    for (int k = 0; k < kNets; ++k)
+    for (int k = 0; k < kBuiltinNets; ++k)
         if (cfg.grid_shares[k] < 0 || cfg.grid_shares[k] > 1024) {
             set_error("grid_shares[%d] must be in [0, 1024]", k);
             return nullptr;
         }
-    for (int k = 0; k < kNets; ++k)
+    for (int k = 0; k < kBuiltinNets; ++k)
         if (cfg.precision[k] > DENGO_FP64_FAST || cfg.precision[k] < -1) {
             set_error("unknown precision %d for network %d", cfg.precision[k], k);
             return nullptr;
         }
-    if (!primordial && !oxygen && !synthetic) {
-        set_error("a server needs at least one network");
-        return nullptr;
-    }
     dengo_server *s = new dengo_server();
     s->cfg = cfg;
     s->precision = precision;
+    for (int k = 0; k < kMaxNets; ++k) s->prec[k] = k < kBuiltinNets ? cfg.precision[k] : -1;
     s->t0 = std::chrono::steady_clock::now();
     s->data[DENGO_NET_PRIMORDIAL] = primordial;
     s->data[DENGO_NET_OXYGEN] = oxygen;
@@ -384,16 +495,9 @@ dengo_server *dengo_server_create(const dengo_server_config *cfg_in, void *primo
     };
     hipError_t e = hipGetDevice(&s->device);
     if (e != hipSuccess) return fail("hipGetDevice", e);
-    for (int k = 0; k < kNets; ++k) {
+    for (int k = 0; k < kBuiltinNets; ++k) {
         if (!s->data[k]) continue;
-        dengo_options o;
-        dengo_default_options(&o);
-        o.precision = s->net_precision(k);
-        o.block = 0;
-        int full = 0;
-        if (s->hooks[k].prepare(s->data[k], &o, &full)) return fail("prepare", hipSuccess);
-        const int share = cfg.grid_shares[k] > 0 ? cfg.grid_shares[k] : cfg.grid_share;
-        s->grid_cap[k] = full / share > 0 ? full / share : 1;
+        if (s->prepare_net(k, cfg.grid_shares[k])) return fail("prepare", hipSuccess);
     }
     if ((e = hipMalloc(&s->arena_base, cfg.hbm_budget)) != hipSuccess)
         return fail("arena hipMalloc", e);
@@ -408,7 +512,9 @@ dengo_server *dengo_server_create(const dengo_server_config *cfg_in, void *primo
         if ((e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking)) != hipSuccess ||
             (e = hipEventCreate(&sl.ev_start)) != hipSuccess ||
             (e = hipEventCreate(&sl.ev_end)) != hipSuccess ||
-            (e = hipEventCreateWithFlags(&sl.ev_done, hipEventDisableTiming)) != hipSuccess)
+            (e = hipEventCreateWithFlags(&sl.ev_done, hipEventDisableTiming)) != hipSuccess ||
+            (e = hipMalloc(&sl.dcounts, 4 * sizeof(int))) != hipSuccess ||
+            (e = hipHostMalloc(&sl.hcounts, 4 * sizeof(int), hipHostMallocDefault)) != hipSuccess)
             return fail("slot set-up", e);
     }
     if ((e = hipEventRecord(s->epoch, s->slots[0].stream)) != hipSuccess ||
@@ -418,18 +524,67 @@ dengo_server *dengo_server_create(const dengo_server_config *cfg_in, void *primo
     return s;
 }
 
-long long dengo_server_submit(dengo_server *s, int network, long long ncells, double *state,
-                              const double *rtol, double *atol, const double *T, int *status,
-                              double dtf, double dt0, double z)
+int dengo_server_add_network(dengo_server *s, void *data, const dengo_net_hooks *hooks,
+                             int precision, int grid_share)
+{
+    if (!s || !data || !hooks) return set_error("null argument");
+    if (precision < -1 || precision > DENGO_FP64_FAST)
+        return set_error("unknown precision %d", precision);
+    if (grid_share < 0 || grid_share > 1024) return set_error("grid_share must be in [0, 1024]");
+    const NetHooks &h = *reinterpret_cast<const NetHooks *>(hooks);
+    if (h.nspecies < 1 || h.nspecies > 1024 || !h.prepare || !h.launch)
+        return set_error("not a network's server hooks");
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev != s->device)
+        return set_error("add the network on the server's device (%d)", s->device);
+    // one registration at a time; the worker reads only registered entries
+    // (ids below nnets), so the new one is filled in without holding mu
+    std::lock_guard<std::mutex> ga(s->add_mu);
+    int id;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (s->stop) return set_error("server is shutting down");
+        if (s->nnets >= kMaxNets) return set_error("a server holds at most %d networks", kMaxNets);
+        id = s->nnets;
+    }
+    s->data[id] = data;
+    s->hooks[id] = h;
+    s->prec[id] = precision;
+    if (s->prepare_net(id, grid_share)) {
+        s->data[id] = nullptr;
+        return set_error("network %d: its tables could not be made resident", id);
+    }
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->nnets = id + 1;
+    }
+    return id;
+}
+
+static long long submit_request(dengo_server *s, int network, long long ncells, double *state,
+                                const double *rtol, double *atol, const double *T, int *status,
+                                double dtf, double dt0, double z, bool device)
 {
     if (!s) return set_error("null server");
-    if (network < 0 || network >= kNets || !s->data[network])
-        return set_error("network %d is not served", network);
     if (ncells <= 0) return set_error("a patch needs at least one cell");
     if (!state || !rtol || !atol) return set_error("null argument");
-    if (s->hooks[network].isothermal && !T)
-        return set_error("the synthetic network needs per-cell temperatures");
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (network < 0 || network >= s->nnets || !s->data[network])
+            return set_error("network %d is not served", network);
+    }
+    const NetHooks &h = s->hooks[network];
+    if (h.isothermal && !T) return set_error("the synthetic network needs per-cell temperatures");
+    if (device) {
+        const size_t row = (size_t)ncells * h.nspecies * sizeof(double);
+        if (check_device_range(state, row, "state") || check_device_range(rtol, row, "rtol") ||
+            check_device_range(atol, row, "atol") ||
+            (h.isothermal && check_device_range(T, (size_t)ncells * sizeof(double), "T")) ||
+            (status && check_device_range(status, (size_t)ncells * sizeof(int), "status")))
+            return -1;
+    }
     Request r;
+    r.device = device;
     r.net = network;
     r.n = ncells;
     r.x = state;
@@ -440,7 +595,11 @@ long long dengo_server_submit(dengo_server *s, int network, long long ncells, do
     r.dtf = dtf;
     r.dt0 = dt0;
     r.z = z;
-    r.bytes = s->request_bytes(network, ncells);
+    r.bytes = s->request_bytes(network, ncells, device);
+    // the densest-first order of a dense host patch is built here, on the
+    // caller's thread, so that the worker never stalls on it
+    if (!device && h.isothermal && s->cfg.dense_order)
+        r.order = density_order(state, ncells, h.nspecies);
     if (r.bytes > s->arena.size())
         return set_error("a patch of %lld cells needs %zu bytes; the arena holds %zu", ncells,
                          r.bytes, s->arena.size());
@@ -455,6 +614,22 @@ long long dengo_server_submit(dengo_server *s, int network, long long ncells, do
     }
     s->cv_work.notify_one();
     return r.ticket;
+}
+
+long long dengo_server_submit(dengo_server *s, int network, long long ncells, double *state,
+                              const double *rtol, double *atol, const double *T, int *status,
+                              double dtf, double dt0, double z)
+{
+    return submit_request(s, network, ncells, state, rtol, atol, T, status, dtf, dt0, z, false);
+}
+
+long long dengo_server_submit_device(dengo_server *s, int network, long long ncells,
+                                     double *d_state, const double *d_rtol, double *d_atol,
+                                     const double *d_T, int *d_status, double dtf, double dt0,
+                                     double z)
+{
+    return submit_request(s, network, ncells, d_state, d_rtol, d_atol, d_T, d_status, dtf, dt0, z,
+                          true);
 }
 
 int dengo_server_poll(dengo_server *s, dengo_patch_result *out, int max)
@@ -523,6 +698,8 @@ void dengo_server_destroy(dengo_server *s)
         if (sl.ev_start) (void)hipEventDestroy(sl.ev_start);
         if (sl.ev_end) (void)hipEventDestroy(sl.ev_end);
         if (sl.ev_done) (void)hipEventDestroy(sl.ev_done);
+        if (sl.dcounts) (void)hipFree(sl.dcounts);
+        if (sl.hcounts) (void)hipHostFree(sl.hcounts);
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
     if (s->epoch) (void)hipEventDestroy(s->epoch);

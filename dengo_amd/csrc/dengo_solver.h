@@ -1613,6 +1613,11 @@ struct Host {
     {                                                                                          \
         return H::evolve_host(d, o, dtf, dt, z, x, rt, at, dims, st, ns, nr, nw, dtc, Tc);     \
     }                                                                                          \
+    const dengo_net_hooks *dengo_##n##_server_hooks(void)                                      \
+    {                                                                                          \
+        static const dengo::NetHooks h = H::hooks();                                           \
+        return reinterpret_cast<const dengo_net_hooks *>(&h);                                  \
+    }                                                                                          \
     int dengo_##n##_set_write_intermediate(n##_data *d, int on)                                \
     {                                                                                          \
         return H::set_write_intermediate(d, on);                                               \

@@ -107,6 +107,7 @@ def _network_signatures(n):
         "temperature_from_mass_density_%s" % n: (None, [_dp, ctypes.c_int, ctypes.c_int, _dp]),
         "dengo_%s_set_current_z" % n: (None, [ctypes.c_void_p, ctypes.c_double]),
         "dengo_%s_set_write_intermediate" % n: (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+        "dengo_%s_server_hooks" % n: (ctypes.c_void_p, []),
         "dengo_%s_strip_temperatures" % n: (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int]),
         "%s_main" % n: (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p]),
         "%s_data_bytes" % n: (ctypes.c_size_t, []),
@@ -170,6 +171,14 @@ SIGNATURES.update({
                                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
                                                 ctypes.c_double, ctypes.c_double]),
+    "dengo_server_submit_device": (ctypes.c_longlong, [ctypes.c_void_p, ctypes.c_int,
+                                                       ctypes.c_longlong, ctypes.c_void_p,
+                                                       ctypes.c_void_p, ctypes.c_void_p,
+                                                       ctypes.c_void_p, ctypes.c_void_p,
+                                                       ctypes.c_double, ctypes.c_double,
+                                                       ctypes.c_double]),
+    "dengo_server_add_network": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                ctypes.c_int, ctypes.c_int]),
     "dengo_server_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PatchResult),
                                          ctypes.c_int]),
     "dengo_server_drain": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
@@ -286,11 +295,15 @@ def density_order(x, nspecies=None):
     (default: every column; pass N - 1 for networks whose last unknown is
     the energy).  It needs no previous solve.  A cell's Newton work grows
     with its density on the mass-action networks, so handing out the costly
-    cells early shortens the persistent schedule's tail.  Returns an int64
+    cells early shortens the persistent schedule's tail.  A NaN total counts
+    as -inf (it sorts last) and ties keep index order, as in the patch
+    server.  Returns an int64
     tensor on x's device."""
     import torch
     xs = x if nspecies is None else x[:, :nspecies]
-    return torch.argsort(xs.sum(dim=1), descending=True, stable=True).contiguous()
+    t = xs.sum(dim=1)
+    key = torch.where(torch.isnan(t), torch.full_like(t, -float("inf")), t)
+    return torch.argsort(key, descending=True, stable=True).contiguous()
 
 
 def write_table_blob(path, block, T_bounds, z_bounds=(0.0, 0.0)):
@@ -633,6 +646,38 @@ def check_patch(nspecies, isothermal, state, rtol, atol, T=None, status=None):
     return [state, rtol, atol] + [a for a in (T, status) if a is not None]
 
 
+def check_device_patch(nspecies, isothermal, state, rtol, atol, T=None, status=None):
+    """The device-tensor form of check_patch: float64 state/rtol/atol of
+    shape (n, N), T float64 (n,) for an isothermal network only, status int32
+    (n,), all contiguous torch tensors on the GPU.  Returns the tensors to keep
+    alive until completion."""
+    import torch
+
+    def need(a, name, dtype, shape):
+        if not (isinstance(a, torch.Tensor) and a.is_cuda):
+            raise ValueError("%s must be a torch tensor on the GPU" % name)
+        if a.dtype != dtype:
+            raise ValueError("%s must be %s, got %s" % (name, dtype, a.dtype))
+        if tuple(a.shape) != shape:
+            raise ValueError("%s must have shape %s, got %s" % (name, shape, tuple(a.shape)))
+        if not a.is_contiguous():
+            raise ValueError("%s must be contiguous" % name)
+    if not (isinstance(state, torch.Tensor) and state.dim() == 2):
+        raise ValueError("state must be a 2-d torch tensor [n, %d]" % nspecies)
+    n = state.shape[0]
+    for a, name in ((state, "state"), (rtol, "rtol"), (atol, "atol")):
+        need(a, name, torch.float64, (n, nspecies))
+    if isothermal:
+        if T is None:
+            raise ValueError("this network needs per-cell temperatures T[n]")
+        need(T, "T", torch.float64, (n,))
+    elif T is not None:
+        raise ValueError("T is only taken by the isothermal network")
+    if status is not None:
+        need(status, "status", torch.int32, (n,))
+    return [state, rtol, atol] + [a for a in (T, status) if a is not None]
+
+
 class PatchServer(object):
     """Continuous batching of patch solves on one GPU (include/dengo_server.h).
 
@@ -660,21 +705,54 @@ class PatchServer(object):
         for k, v in (net_grid_share or {}).items():
             cfg.grid_shares[k] = v
         cfg.dense_order = 1 if dense_order else 0
-        self._solvers = (primordial, oxygen, synthetic)
-        ptr = [None if x is None else x.data for x in self._solvers]
+        self._solvers = {k: v for k, v in enumerate((primordial, oxygen, synthetic))
+                         if v is not None}
+        ptr = [None if x is None else x.data for x in (primordial, oxygen, synthetic)]
         self.srv = lib.dengo_server_create(ctypes.byref(cfg), ptr[0], ptr[1], ptr[2], precision)
         if not self.srv:
             raise RuntimeError("dengo_server_create failed: %s" % last_error())
         self._keep = {}
 
+    def add_network(self, solver, precision=-1, grid_share=0):
+        """Serves another network: `solver` is a NetworkSolver of any solver
+        library (a GeneratedSolver of a write_solver output, for instance).
+        Returns the network id to submit with."""
+        if not isinstance(solver, NetworkSolver):
+            raise ValueError("add_network takes a NetworkSolver")
+        hooks = solver._fn("dengo_%s_server_hooks")()
+        k = load().dengo_server_add_network(self.srv, solver.data, hooks, precision, grid_share)
+        if k < 0:
+            raise RuntimeError("dengo_server_add_network failed: %s" % last_error())
+        self._solvers[k] = solver
+        return k
+
+    def _solver(self, network):
+        if network not in self._solvers:
+            raise ValueError("network %r is not served" % (network,))
+        return self._solvers[network]
+
+    def submit_device(self, network, state, rtol, atol, dtf, T=None, status=None, dt0=-1.0,
+                      z=-1.0):
+        """Queue one patch held in GPU torch tensors, integrated in place
+        (nothing is copied; the tensors must not be touched until the
+        completion record is polled).  Returns the ticket."""
+        solver = self._solver(network)
+        keep = check_device_patch(solver.N, isinstance(solver, SyntheticSolver), state, rtol,
+                                  atol, T, status)
+        ptr = lambda a: None if a is None else a.data_ptr()
+        t = load().dengo_server_submit_device(
+            self.srv, network, state.shape[0], ptr(state), ptr(rtol), ptr(atol), ptr(T),
+            ptr(status), dtf, dt0, z)
+        if t < 0:
+            raise RuntimeError("dengo_server_submit_device failed: %s" % last_error())
+        self._keep[t] = keep
+        return t
+
     def submit(self, network, state, rtol, atol, dtf, T=None, status=None, dt0=-1.0, z=-1.0):
         """Queue one patch; state and atol [n, N] are updated in place at
         completion, status[n] (int32) receives per-cell status.  Returns the
         ticket."""
-        if network not in (NET_PRIMORDIAL, NET_OXYGEN, NET_SYNTHETIC) or \
-                self._solvers[network] is None:
-            raise ValueError("network %r is not served" % (network,))
-        solver = self._solvers[network]
+        solver = self._solver(network)
         arrays = check_patch(solver.N, isinstance(solver, SyntheticSolver), state, rtol, atol, T,
                              status)
         n = state.shape[0]

@@ -38,7 +38,8 @@
 extern "C" {
 #endif
 
-/* Networks a server can hold (one data struct each, any may be NULL). */
+/* Networks a server holds from its creation (one data struct each, any may be
+ * NULL); dengo_server_add_network adds others (ids 3, 4, ..., at most 16 in all). */
 enum dengo_network { DENGO_NET_PRIMORDIAL = 0, DENGO_NET_OXYGEN = 1, DENGO_NET_SYNTHETIC = 2 };
 
 typedef struct dengo_server dengo_server;
@@ -92,7 +93,8 @@ typedef struct dengo_server_stats {
 void dengo_server_default_config(dengo_server_config *cfg);
 
 /* Creates a server on the current device for the given data structs (tables
- * set; NULL for a network it will not serve).  precision selects the Newton
+ * set; NULL for a network it will not serve; all three may be NULL when the
+ * networks are added afterwards).  precision selects the Newton
  * sweep of each network's patches unless cfg->precision overrides it for that
  * network (enum dengo_precision; the dense synthetic network supports
  * DENGO_FP64 and DENGO_FP64_FAST, and DENGO_MIXED falls back to
@@ -110,6 +112,29 @@ dengo_server *dengo_server_create(const dengo_server_config *cfg, void *primordi
 long long dengo_server_submit(dengo_server *srv, int network, long long ncells, double *state,
         const double *rtol, double *atol, const double *T, int *status, double dtf, double dt0,
         double z);
+
+/* Queues one patch held in device memory of the server's device: d_state,
+ * d_rtol, d_atol [ncells][N], d_T[ncells] (isothermal network only) and the
+ * optional d_status[ncells] are the caller's, integrated in place with nothing
+ * copied (no arena space is used), so a patch's state can stay on the device
+ * between its successive requests.  The arrays must stay valid and untouched
+ * until the request completes; its completion record carries the status
+ * counts.  Each pointer is checked to be device memory that holds the patch
+ * (hipPointerGetAttributes, hipMemGetAddressRange).  The dense network's cells
+ * are handed out in index order here.  Returns a ticket >= 0, or a negative
+ * code. */
+long long dengo_server_submit_device(dengo_server *srv, int network, long long ncells,
+        double *d_state, const double *d_rtol, double *d_atol, const double *d_T, int *d_status,
+        double dtf, double dt0, double z);
+
+/* Adds a network to a running server: `data` is its data struct (tables set)
+ * and `hooks` the dengo_<n>_server_hooks() of the solver library that holds
+ * it (any library write_solver emits, loaded in this process).  precision is
+ * its sweep's precision (-1: the server's), grid_share its patches' share of
+ * the GPU (0: cfg->grid_share).  Call it on the server's device.  Returns the
+ * network id to submit with (>= 3), or a negative code. */
+int dengo_server_add_network(dengo_server *srv, void *data, const dengo_net_hooks *hooks,
+        int precision, int grid_share);
 
 /* Moves up to max completion records into out (completion order); returns
  * how many. */

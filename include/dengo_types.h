@@ -27,6 +27,10 @@ enum dengo_schedule {
     DENGO_PERSISTENT = 1  /* resident lanes pull cells from a device queue */
 };
 
+/* A network's hooks for the patch server (opaque; each solver library's
+   dengo_<n>_server_hooks returns its own, for dengo_server_add_network). */
+typedef struct dengo_net_hooks dengo_net_hooks;
+
 typedef struct dengo_options {
     int precision;        /* enum dengo_precision */
     int schedule;         /* enum dengo_schedule */

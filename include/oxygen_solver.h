@@ -235,6 +235,9 @@ void temperature_from_mass_density_oxygen(double *input, int nstrip, int nchem,
    attributes; a cell with fewer substeps repeats its last record (NaN without
    one).  Returns 0, or a negative error code. */
 int dengo_oxygen_set_write_intermediate(oxygen_data *data, int on);
+/* This network's hooks for dengo_server_add_network (include/dengo_server.h):
+   a patch server in the same process can then serve it. */
+const dengo_net_hooks *dengo_oxygen_server_hooks(void);
 /* Sets data->current_z, the redshift the strip solve uses. */
 void dengo_oxygen_set_current_z(oxygen_data *data, double z);
 /* Copies data->Ts (temperatures of the last strip RHS evaluation) for

@@ -315,6 +315,9 @@ void temperature_from_mass_density_primordial(double *input, int nstrip, int nch
    attributes; a cell with fewer substeps repeats its last record (NaN without
    one).  Returns 0, or a negative error code. */
 int dengo_primordial_set_write_intermediate(primordial_data *data, int on);
+/* This network's hooks for dengo_server_add_network (include/dengo_server.h):
+   a patch server in the same process can then serve it. */
+const dengo_net_hooks *dengo_primordial_server_hooks(void);
 /* Sets data->current_z, the redshift the strip solve uses. */
 void dengo_primordial_set_current_z(primordial_data *data, double z);
 /* Copies data->Ts (temperatures of the last strip RHS evaluation) for

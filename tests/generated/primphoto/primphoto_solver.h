@@ -232,6 +232,9 @@ void temperature_from_mass_density(double *input, int nstrip, int nchem,
    attributes; a cell with fewer substeps repeats its last record (NaN without
    one).  Returns 0, or a negative error code. */
 int dengo_primphoto_set_write_intermediate(primphoto_data *data, int on);
+/* This network's hooks for dengo_server_add_network (include/dengo_server.h):
+   a patch server in the same process can then serve it. */
+const dengo_net_hooks *dengo_primphoto_server_hooks(void);
 /* Sets data->current_z, the redshift the strip solve uses. */
 void dengo_primphoto_set_current_z(primphoto_data *data, double z);
 /* Copies data->Ts (temperatures of the last strip RHS evaluation) for

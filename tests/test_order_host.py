@@ -26,3 +26,12 @@ def test_orders_keep_index_order_on_ties():
     assert runtime.density_order(x).tolist() == [4, 0, 1, 2, 3, 5, 6]
     sw = torch.tensor([3, 9, 3, 1, 9, 0], dtype=torch.int32)
     assert runtime.cost_order(sw).tolist() == [1, 4, 0, 2, 3, 5]
+
+
+def test_density_order_puts_nan_totals_last():
+    """A NaN total counts as -inf, so it sorts last (tied with a total of
+    -inf), ties in index order: the patch server's rule for its dense
+    patches."""
+    x = torch.tensor([[1.0], [float("nan")], [3.0], [-float("inf")], [float("nan")], [3.0]],
+                     dtype=torch.float64)
+    assert runtime.density_order(x).tolist() == [2, 5, 0, 1, 3, 4]

@@ -42,8 +42,6 @@ def test_server_create_rejects_bad_arguments():
     assert not lib.dengo_server_create(cfg, None, None, None, runtime.MIXED)
     assert "slots" in runtime.last_error()
     lib.dengo_server_default_config(cfg)
-    assert not lib.dengo_server_create(cfg, None, None, None, runtime.MIXED)
-    assert "at least one network" in runtime.last_error()
     cfg.precision[1] = 7
     assert not lib.dengo_server_create(cfg, None, None, None, runtime.MIXED)
     assert "unknown precision 7 for network 1" in runtime.last_error()
