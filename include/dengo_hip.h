@@ -8,6 +8,8 @@
  *   calculate_rhs_<n>     <- :460-522
  *   calculate_jacobian_<n> <- :527-599
  *   <n>_calculate_temperature <- :309-350
+ *   ensure_electron_consistency_<n>, temperature_from_mass_density_<n> <- :604-668
+ *   <n>_main              <- :56-147
  * plus the backend extensions declared in <n>_solver.h.
  *
  * Semantics of the drop-in: every cell is integrated as its own strip of one
@@ -15,9 +17,10 @@
  * convergence test and its own electron total.  The return value is the
  * smallest elapsed time over cells (< dtf means some cell did not finish, the
  * reference's only failure signal) and *dt is that cell's final substep.
- * Tables are read from <n>_tables.bin in the working directory (a 64-byte
- * header "DENGOTB1", int32 ntables, int32 nT, double T0, T1, z0, z1, then
- * FP64 [ntables][nT]); the reference reads <n>_tables.h5.
+ * <n>_setup_data reads the reference's <n>_tables.h5 from the working
+ * directory (libhdf5_hl opened at run time); without that file or without
+ * HDF5 it reads <n>_tables.bin (a 64-byte header "DENGOTB1", int32 ntables,
+ * int32 nT, double T0, T1, z0, z1, then FP64 [ntables][nT]).
  */
 #ifndef DENGO_HIP_H
 #define DENGO_HIP_H
