@@ -64,8 +64,11 @@ def serving_leg(args, prim):
                                                             sum(t[2] for t in trace)))
     patches, keep = [], []
     for (_, k, n, s) in trace:
-        p, kp = serving.make_patch(k, n, s, synthetic_net=net)
-        patches.append(p)
+        p, kp = serving.make_patch(k, n, s, synthetic_net=net, pinned=args.serve_host)
+        # resident (default): the patches' state lives in device memory between
+        # requests, as a hydro code on the GPU keeps it; each request is
+        # integrated in place
+        patches.append(p if args.serve_host else serving.to_device(p))
         keep.append(kp)
     # precision per network: "auto" lets the autotuner pick, on a sample of
     # cells of the patches' distributions, the fastest sweep that meets the
@@ -120,7 +123,12 @@ def serving_leg(args, prim):
                     serving.NET_NAMES[k]: {"precision_trials": v["precision_trials"],
                                            "sample": v["sample"]} for k, v in tuned.items()},
                 "arena_peak_bytes": stats["arena_peak"], "arena_bytes": stats["arena_bytes"],
-                "latency": "submit (host arrays in pinned memory) to copy-back complete"})
+                "state": "host arrays in pinned memory, copied in and out per request"
+                         if args.serve_host else
+                         "device-resident: each patch's state stays in device memory across "
+                         "requests and is integrated in place (dengo_server_submit_device)",
+                "latency": "submit to copy-back complete" if args.serve_host else
+                           "submit to kernel and status counts complete"})
     log("serving: p50 %.1f ms, p99 %.1f ms, %.0f cell-solves/s" % (
         out["latency_ms_p50"], out["latency_ms_p99"], out["cell_solves_per_s"] or 0))
     for kp in keep:
@@ -272,6 +280,9 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=15.0,
                     help="patch server: arrival window (s); 0 skips the serving leg")
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-host", action="store_true",
+                    help="patch server: patches as pinned host arrays copied in and out per "
+                         "request (default: device-resident patches integrated in place)")
     ap.add_argument("--serve-precision", default="auto", choices=["auto", "parity", "mixed"],
                     help="auto: per network, the autotuner's fastest sweep that meets the 1e-6 "
                          "bar on a sample; parity: mixed for oxygen, FP64-fast elsewhere; "

@@ -21,6 +21,8 @@
 #include <thread>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "arena.h"
 #include "dengo_server.h"
 #include "dengo_solver.h"
@@ -49,6 +51,7 @@ struct Request {
     size_t bytes = 0;
     int bypassed = 0;
     std::vector<long long> order;   // dense network, host submits: densest first
+    size_t sort_temp = 0;           // dense network, device submits: the sort's scratch
 };
 
 // Densest cells first: indices by descending total of the n x ns species
@@ -114,6 +117,33 @@ __global__ void __launch_bounds__(256) k_status_counts(const int *status, long l
     if (threadIdx.x < 3) out[threadIdx.x] = c[threadIdx.x][0];
 }
 
+// Sort keys of a device-resident dense patch: each cell's total density (the
+// species columns summed in index order, as density_order does on the host; a
+// NaN total counts as -inf) and its index.
+template <int>
+__global__ void __launch_bounds__(256) k_density_keys(const double *x, long long n, int ns,
+                                                      double *key, long long *idx)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double t = 0.0;
+    for (int j = 0; j < ns; ++j) t += x[i * ns + j];
+    key[i] = isnan(t) ? -HUGE_VAL : t;
+    idx[i] = i;
+}
+
+// Scratch bytes of the stable descending radix sort of n (key, index) pairs.
+size_t sort_temp_bytes(long long n)
+{
+    size_t b = 0;
+    if (hipcub::DeviceRadixSort::SortPairsDescending(nullptr, b, (const double *)nullptr,
+                                                     (double *)nullptr,
+                                                     (const long long *)nullptr,
+                                                     (long long *)nullptr, (int)n) != hipSuccess)
+        return 0;
+    return b;
+}
+
 // A device pointer the caller hands over must be device memory of at least
 // `bytes` from ptr on; anything else would fault the kernel.
 int check_device_range(const void *ptr, size_t bytes, const char *what)
@@ -177,9 +207,15 @@ struct dengo_server {
         return pr;
     }
 
-    size_t request_bytes(int net, long long n, bool device) const
+    size_t request_bytes(int net, long long n, bool device, size_t sort_temp) const
     {
-        if (device) return 0;   // integrated in the caller's buffers
+        // a device patch is integrated in the caller's buffers; a dense one
+        // still needs its densest-first order: keys in and out, indices in,
+        // the order, and the sort's scratch
+        if (device)
+            return hooks[net].isothermal && cfg.dense_order
+                       ? 4 * round_up((size_t)n * sizeof(double)) + round_up(sort_temp)
+                       : 0;
         const size_t row = (size_t)n * hooks[net].nspecies * sizeof(double);
         size_t b = 3 * round_up(row) + round_up((size_t)n * sizeof(int));
         // the dense network's patches also carry T[n] and the order[n]
@@ -255,7 +291,25 @@ struct dengo_server {
     {
         const Request &q = s.req;
         const size_t row = (size_t)q.n * hooks[q.net].nspecies * sizeof(double);
-        if (q.device) return enqueue_kernel(s);
+        if (q.device) {
+            if (s.dord) {
+                // densest cells first, sorted on the slot's stream (stable:
+                // equal totals keep index order, as on the host)
+                const size_t nb = round_up((size_t)q.n * sizeof(double));
+                char *p = arena_base + s.off;
+                double *kin = (double *)p, *kout = (double *)(p + nb);
+                long long *iin = (long long *)(p + 2 * nb);
+                void *tmp = p + 4 * nb;
+                const unsigned blocks = (unsigned)((q.n + 255) / 256);
+                hipLaunchKernelGGL(k_density_keys<0>, dim3(blocks), dim3(256), 0, s.stream, q.x,
+                                   q.n, hooks[q.net].nspecies, kin, iin);
+                DENGO_HIP_OK(hipGetLastError());
+                size_t tb = q.sort_temp;
+                DENGO_HIP_OK(hipcub::DeviceRadixSort::SortPairsDescending(
+                    tmp, tb, kin, kout, iin, s.dord, (int)q.n, 0, 64, s.stream));
+            }
+            return enqueue_kernel(s);
+        }
         DENGO_HIP_OK(hipMemcpyAsync(s.dx, q.x, row, hipMemcpyHostToDevice, s.stream));
         DENGO_HIP_OK(hipMemcpyAsync(s.drt, q.rtol, row, hipMemcpyHostToDevice, s.stream));
         DENGO_HIP_OK(hipMemcpyAsync(s.dat, q.atol, row, hipMemcpyHostToDevice, s.stream));
@@ -376,7 +430,7 @@ struct dengo_server {
                     }
                 if (!fs) break;
                 const int idx = pick_admission(
-                    waiting, [&](const Request &r) { return r.device || arena.fits(r.bytes); },
+                    waiting, [&](const Request &r) { return r.bytes == 0 || arena.fits(r.bytes); },
                     cfg.max_bypass);
                 if (idx < 0) break;
                 Slot &s = *fs;
@@ -386,13 +440,15 @@ struct dengo_server {
                     // the caller's buffers; the dense network's cells keep
                     // index order (the densest-first order is built from host
                     // state)
-                    s.off = -1;
+                    s.off = s.req.bytes ? arena.alloc(s.req.bytes) : -1;
                     s.dx = s.req.x;
                     s.drt = const_cast<double *>(s.req.rtol);
                     s.dat = s.req.atol;
                     s.dst = s.req.status;
                     s.dT = const_cast<double *>(s.req.T);
-                    s.dord = nullptr;
+                    s.dord = s.req.bytes ? (long long *)(arena_base + s.off +
+                                                         3 * round_up((size_t)s.req.n * 8))
+                                         : nullptr;
                 } else {
                     s.off = arena.alloc(s.req.bytes);
                     char *p = arena_base + s.off;
@@ -595,7 +651,12 @@ static long long submit_request(dengo_server *s, int network, long long ncells, 
     r.dtf = dtf;
     r.dt0 = dt0;
     r.z = z;
-    r.bytes = s->request_bytes(network, ncells, device);
+    if (device && h.isothermal && s->cfg.dense_order) {
+        if (ncells > 0x7fffffffLL) return set_error("a device patch holds at most 2^31 - 1 cells");
+        r.sort_temp = sort_temp_bytes(ncells);
+        if (r.sort_temp == 0) return set_error("cannot size the cell sort of %lld cells", ncells);
+    }
+    r.bytes = s->request_bytes(network, ncells, device, r.sort_temp);
     // the densest-first order of a dense host patch is built here, on the
     // caller's thread, so that the worker never stalls on it
     if (!device && h.isothermal && s->cfg.dense_order)

@@ -8,7 +8,8 @@ seeded distributions as the single-network workloads.  Arrival gaps are
 exponential at a fixed rate.  The replay submits each patch at its arrival
 time (host clock of the server) and collects the completion records; latency
 is completion minus arrival, throughput is cells completed over the span from
-the first arrival to the last completion.
+the first arrival to the last completion.  Patches are host arrays (copied in
+and out per request) or device tensors (resident; integrated in place).
 """
 import math
 import time
@@ -73,13 +74,23 @@ def make_patch(net, ncells, seed, synthetic_net=None, pinned=True):
     return out, keep
 
 
+def to_device(patch, device="cuda:0"):
+    """The patch's arrays as GPU torch tensors: the state of a hydro code that
+    keeps its patches resident in device memory between requests."""
+    import torch
+    return {k: None if a is None else torch.from_numpy(np.ascontiguousarray(a)).to(device)
+            for k, a in patch.items()}
+
+
 def percentile(v, q):
     return float(np.percentile(np.asarray(v, dtype=float), q)) if len(v) else None
 
 
 def replay(server, trace, patches, dtf, timeout_s=600.0):
     """Submits patches[i] at trace[i]'s arrival time (relative to the call)
-    and waits for all of them; returns the completion records (arrival order)."""
+    and waits for all of them; returns the completion records (arrival order).
+    A patch of torch tensors (to_device) goes through submit_device and is
+    integrated in place; a patch of host arrays is copied in and out."""
     t0 = server.now()
     tickets = []
     for (ta, net, n, _), p in zip(trace, patches):
@@ -88,8 +99,9 @@ def replay(server, trace, patches, dtf, timeout_s=600.0):
             if lag <= 0:
                 break
             time.sleep(min(lag, 0.002))
-        tickets.append(server.submit(net, p["state"], p["rtol"], p["atol"], dtf, T=p["T"],
-                                     status=p["status"]))
+        submit = server.submit if isinstance(p["state"], np.ndarray) else server.submit_device
+        tickets.append(submit(net, p["state"], p["rtol"], p["atol"], dtf, T=p["T"],
+                              status=p["status"]))
     server.drain(timeout_s)
     recs = {r["ticket"]: r for r in server.poll(len(tickets) + 16)}
     return [recs[t] for t in tickets], t0

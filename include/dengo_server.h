@@ -120,9 +120,11 @@ long long dengo_server_submit(dengo_server *srv, int network, long long ncells, 
  * between its successive requests.  The arrays must stay valid and untouched
  * until the request completes; its completion record carries the status
  * counts.  Each pointer is checked to be device memory that holds the patch
- * (hipPointerGetAttributes, hipMemGetAddressRange).  The dense network's cells
- * are handed out in index order here.  Returns a ticket >= 0, or a negative
- * code. */
+ * (hipPointerGetAttributes, hipMemGetAddressRange).  With cfg->dense_order, a
+ * dense-network patch's cells are handed out densest first as for host
+ * patches; the order is sorted on the device in arena scratch (4 x 8 bytes per
+ * cell plus the sort's own), the only arena space a device patch takes.
+ * Returns a ticket >= 0, or a negative code. */
 long long dengo_server_submit_device(dengo_server *srv, int network, long long ncells,
         double *d_state, const double *d_rtol, double *d_atol, const double *d_T, int *d_status,
         double dtf, double dt0, double z);

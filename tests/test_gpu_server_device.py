@@ -10,7 +10,9 @@
   its patches equal the carbon library's own evolve bit for bit.
 * Device submits are refused, before anything is queued, for host memory, for
   an allocation smaller than the patch, and for a network the server does not
-  hold."""
+  hold.
+* A dense-network device patch gets its densest-first order sorted on the
+  device and equals the synthetic library's own evolve."""
 import ctypes
 import importlib.util
 import os
@@ -110,6 +112,39 @@ def test_generated_network_served_in_the_mix():
         srv.close()
         carbon.close()
         ps.close()
+
+
+def test_dense_device_patch_densest_first():
+    """A dense-network patch in device memory: its densest-first order is
+    sorted on the device (the server's cfg dense_order), and its cells equal
+    the same cells through the synthetic library's host entry point."""
+    print("server: dense device patch, order sorted on the device", flush=True)
+    from dengo_amd.synthetic import SyntheticNetwork
+    syn = runtime.SyntheticSolver(SyntheticNetwork(seed=7))
+    srv = runtime.PatchServer(synthetic=syn, precision=runtime.FP64_FAST, slots=2,
+                              hbm_budget=1 << 26)
+    try:
+        g = np.load(os.path.join(ROOT, "tests", "golden", "synthetic_cells.npz"))
+        sl = np.r_[0:6, 12:17]
+        x, rt, at, T = (np.ascontiguousarray(g[k][sl]) for k in ("input", "rtol", "atol", "T"))
+        dtf = float(g["dtf"])
+        # scramble the densities' order so that the sort has work to do
+        perm = np.random.default_rng(5).permutation(x.shape[0])
+        x, rt, at, T = x[perm].copy(), rt[perm].copy(), at[perm].copy(), T[perm].copy()
+        ref = syn.evolve(x, T, rt, at, dtf, precision=runtime.FP64_FAST)
+        xd, atd = torch.from_numpy(x).cuda(), torch.from_numpy(at).cuda()
+        st = torch.zeros(x.shape[0], dtype=torch.int32, device="cuda")
+        t = srv.submit_device(runtime.NET_SYNTHETIC, xd, torch.from_numpy(rt).cuda(), atd, dtf,
+                              T=torch.from_numpy(T).cuda(), status=st)
+        rec = _done(srv, [t])[t]
+        assert np.array_equal(xd.cpu().numpy(), ref["state"])
+        assert np.array_equal(atd.cpu().numpy(), ref["atol"])
+        assert np.array_equal(st.cpu().numpy(), ref["status"])
+        assert list(rec["status_counts"]) == list(np.bincount(ref["status"], minlength=3))
+        assert srv.stats()["arena_peak"] > 0       # the sort's keys and scratch
+    finally:
+        srv.close()
+        syn.close()
 
 
 def test_device_submit_refusals():
