@@ -180,7 +180,7 @@ def parity_stats(x, st, nsub, nrej, x_ref, st_ref, nsub_ref, nrej_ref):
 
 
 def network_leg(name, solver, x0, rtol, atol, variants, dtf=1e12, T=None, flops_per_sweep=None,
-                ordered=(), density_ordered=(), nspecies=None):
+                ordered=(), density_ordered=(), nspecies=None, graph=False):
     """Runs every variant once on device copies of the same inputs; the first
     variant is the parity reference.  Returns per-variant kernel time,
     cell-solves/s and parity statistics.  Each variant named in `ordered` runs
@@ -190,7 +190,9 @@ def network_leg(name, solver, x0, rtol, atol, variants, dtf=1e12, T=None, flops_
     against the variant's own unordered run.  Each variant named in
     `density_ordered` reruns with the densest cells first
     (runtime.density_order over the first `nspecies` columns), an order that
-    needs no previous solve."""
+    needs no previous solve.  With graph, every run is one replay of a
+    captured HIP graph of the integration (dengo_graph_create/launch), timed
+    with events on the stream it is launched on."""
     import torch
     from dengo_amd import runtime
     dev = torch.device("cuda:0")
@@ -203,6 +205,20 @@ def network_leg(name, solver, x0, rtol, atol, variants, dtf=1e12, T=None, flops_
         at = torch.from_numpy(atol).to(dev)
         o = {k: torch.zeros(n, dtype=torch.int32, device=dev)
              for k in ("status", "nsub", "nrej", "nsweep")}
+        if graph:
+            g = runtime.CapturedEvolve(solver, x, d_rt, at, dtf,
+                                       T=None if T is None else torch.from_numpy(T).to(dev),
+                                       outputs=o, precision=prec, order=order)
+            stream = torch.cuda.Stream()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record(stream)
+            g.launch(stream)
+            e1.record(stream)
+            e1.synchronize()
+            g.close()
+            return e0.elapsed_time(e1), x, o
         if T is None:
             solver.evolve_device(x, d_rt, at, dtf, outputs=o, precision=prec, order=order)
         else:
@@ -214,6 +230,7 @@ def network_leg(name, solver, x0, rtol, atol, variants, dtf=1e12, T=None, flops_
         ms, x, o = solve(prec)
         sw = o["nsweep"].double()
         rec = {"kernel_ms": ms, "cell_solves_per_s": n / (ms * 1e-3),
+               "launch": "captured HIP graph replay" if graph else "evolve_device",
                "status_counts": torch.bincount(o["status"].long(), minlength=3).tolist(),
                "cell_sweeps_p99_9": float(torch.quantile(sw[:1 << 24], 0.999)) if n else 0.0,
                "cell_sweeps_max": int(o["nsweep"].max()) if n else 0}
@@ -472,7 +489,8 @@ def main():
         rts, ats = tolerances(xs)
         workloads["synthetic"] = {
             "workload": "synthetic 128 species / 1500 reactions, dense Jacobian, %d cells, "
-                        "dtf=1e12 s" % args.synthetic_cells,
+                        "dtf=1e12 s, each run one replay of a captured HIP graph"
+                        % args.synthetic_cells,
             "variants": network_leg("synthetic", syn, xs, rts, ats,
                                     [("fp64fast_persistent", runtime.FP64_FAST)], T=Ts,
                                     flops_per_sweep=flopcount.dense_sweep_flops(
@@ -480,7 +498,7 @@ def main():
                                     ordered=() if args.no_cost_ordered
                                     else ("fp64fast_persistent",),
                                     density_ordered=() if args.no_cost_ordered
-                                    else ("fp64fast_persistent",))}
+                                    else ("fp64fast_persistent",), graph=True)}
         syn.close()
         del xs, Ts, rts, ats
 
