@@ -445,16 +445,20 @@ struct DevState {
     double *d_u0 = nullptr, *d_sc = nullptr, *d_inv = nullptr, *d_unew = nullptr;
     int *d_code = nullptr;
     OrderBufs ord;                     // cell order: rows and outputs in hand-out order
-    // intermediate solutions: -1 follows the network's WRITE_INTERMEDIATE; the
-    // records exist only while a host-array evolve that writes them runs
+    // intermediate solutions: -1 follows the network's WRITE_INTERMEDIATE
     int write_im = -1;
-    double *d_traj = nullptr;
-    long long traj_cap = 0;
 };
 
 // Device memory for intermediate-solution records, (N + 3) doubles per cell
 // and accepted substep: the records stop at the substep this budget reaches.
 constexpr size_t kIntermediateBytes = size_t(1) << 30;
+
+// Intermediate-solution records of one host-array evolve (KParams::traj),
+// handed down its own call chain only, never through shared state.
+struct Records {
+    double *d = nullptr;
+    long long cap = 0;
+};
 
 inline int check_options(const dengo_options *opt, dengo_options *o)
 {
@@ -941,11 +945,12 @@ struct Host {
     static int launch(Data *d, DevState *s, const dengo_options &o, double dtf, double dt0,
                       double z, double *d_input, const double *d_rtol, double *d_atol,
                       long long dims, const dengo_cell_outputs &out, hipStream_t stream,
-                      unsigned long long *counter, bool events, OrderBufs *ob = nullptr)
+                      unsigned long long *counter, bool events, OrderBufs *ob = nullptr,
+                      const Records *rec = nullptr)
     {
         if (!o.order || dims <= 0)
             return launch_cells(d, s, o, dtf, dt0, z, d_input, d_rtol, d_atol, dims, out, stream,
-                                counter, events);
+                                counter, events, rec);
         // cells in hand-out order: gather, integrate, scatter back (the
         // buffers were allocated and the order checked by the caller)
         OrderBufs &b = ob ? *ob : s->ord;
@@ -975,7 +980,8 @@ struct Host {
     static int launch_cells(Data *d, DevState *s, const dengo_options &o, double dtf, double dt0,
                             double z, double *d_input, const double *d_rtol, double *d_atol,
                             long long dims, const dengo_cell_outputs &out, hipStream_t stream,
-                            unsigned long long *counter, bool events)
+                            unsigned long long *counter, bool events,
+                            const Records *rec = nullptr)
     {
         KParams<Net> p;
         p.input = d_input;
@@ -996,8 +1002,8 @@ struct Host {
         p.T_out = out.T;
         p.counter = counter;
         // intermediate records come from the reference-faithful sweep only
-        p.traj = o.precision == DENGO_FP64 ? s->d_traj : nullptr;
-        p.traj_cap = s->traj_cap;
+        p.traj = rec && o.precision == DENGO_FP64 ? rec->d : nullptr;
+        p.traj_cap = rec ? rec->cap : 0;
         if (dims <= 0) return 0;
         void (*kp)(KParams<Net>) = nullptr;
         void (*kc)(KParams<Net>) = nullptr;
@@ -1026,7 +1032,8 @@ struct Host {
     static int evolve_device(Data *d, const dengo_options *opt, double dtf, double dt0, double z,
                              double *d_input, const double *d_rtol, double *d_atol,
                              long long dims, const dengo_cell_outputs *outputs,
-                             double *ttot_min, double *dt_min, void *stream)
+                             double *ttot_min, double *dt_min, void *stream,
+                             const Records *rec = nullptr)
     {
         if (!d || !d_input || !d_rtol || !d_atol) return set_error("null argument");
         if (dims < 0) return set_error("negative cell count");
@@ -1047,7 +1054,8 @@ struct Host {
             (order_alloc(s->ord, dims, N) || order_check(s->ord, o.order, dims, st)))
             return -1;
         d->current_z = z;
-        if (launch(d, s, o, dtf, dt0, z, d_input, d_rtol, d_atol, dims, out, st, s->d_counter, true))
+        if (launch(d, s, o, dtf, dt0, z, d_input, d_rtol, d_atol, dims, out, st, s->d_counter, true,
+                   nullptr, rec))
             return -1;
         if (ttot_min) {
             if (dims > 0) {
@@ -1082,12 +1090,11 @@ struct Host {
         DevState *s = state(d);
         if (ensure_device(d, s) || ensure_scratch(s, dims)) return -1.0;
         if (writes_intermediate(s)) {
+            Records rec;
             const double t = evolve_host_intermediate(d, s, opt, dtf, dt, z, input, rtol, atol,
                                                       dims, status, nsub, nrej, nsweep, dt_cell,
-                                                      T_cell);
-            if (s->d_traj) (void)hipFree(s->d_traj);
-            s->d_traj = nullptr;
-            s->traj_cap = 0;
+                                                      T_cell, rec);
+            if (rec.d) (void)hipFree(rec.d);
             return t;
         }
         return evolve_host_plain(d, s, opt, dtf, dt, z, input, rtol, atol, dims, status, nsub,
@@ -1118,7 +1125,7 @@ struct Host {
                                            double dtf, double *dt, double z, double *input,
                                            double *rtol, double *atol, long long dims,
                                            int *status, int *nsub, int *nrej, int *nsweep,
-                                           double *dt_cell, double *T_cell)
+                                           double *dt_cell, double *T_cell, Records &rc)
     {
         dengo_options o;
         if (check_options(opt, &o)) return -1.0;
@@ -1132,15 +1139,15 @@ struct Host {
                       kIntermediateBytes);
             return -1.0;
         }
-        if (hipMalloc(&s->d_traj, (size_t)cap * rec) != hipSuccess) {
-            s->d_traj = nullptr;
+        if (hipMalloc(&rc.d, (size_t)cap * rec) != hipSuccess) {
+            rc.d = nullptr;
             set_error("cannot allocate %zu bytes of intermediate records", (size_t)cap * rec);
             return -1.0;
         }
-        s->traj_cap = cap;
+        rc.cap = cap;
         std::vector<int> ns((size_t)dims);
         const double ttot = evolve_host_plain(d, s, &o, dtf, dt, z, input, rtol, atol, dims, status,
-                                              ns.data(), nrej, nsweep, dt_cell, T_cell);
+                                              ns.data(), nrej, nsweep, dt_cell, T_cell, &rc);
         if (ttot < 0) return ttot;
         if (nsub) memcpy(nsub, ns.data(), (size_t)dims * sizeof(int));
         long long K = 0;
@@ -1152,7 +1159,7 @@ struct Host {
         }
         std::vector<double> recs((size_t)K * dims * (N + 3));
         if (K > 0 &&
-            hipMemcpy(recs.data(), s->d_traj, recs.size() * sizeof(double),
+            hipMemcpy(recs.data(), rc.d, recs.size() * sizeof(double),
                       hipMemcpyDeviceToHost) != hipSuccess) {
             set_error("device-to-host copy failed");
             return -1.0;
@@ -1217,7 +1224,8 @@ struct Host {
     static double evolve_host_plain(Data *d, DevState *s, const dengo_options *opt, double dtf,
                                     double *dt, double z, double *input, double *rtol,
                                     double *atol, long long dims, int *status, int *nsub,
-                                    int *nrej, int *nsweep, double *dt_cell, double *T_cell)
+                                    int *nrej, int *nsweep, double *dt_cell, double *T_cell,
+                                    const Records *rec = nullptr)
     {
         const size_t bytes = (size_t)dims * N * sizeof(double);
         if (hipMemcpy(s->d_in, input, bytes, hipMemcpyHostToDevice) != hipSuccess ||
@@ -1236,7 +1244,7 @@ struct Host {
         out.T = s->d_T;
         double tmin = 0.0, dtm = 0.0;
         if (evolve_device(d, opt, dtf, *dt, z, s->d_in, s->d_rtol, s->d_atol, dims, &out, &tmin,
-                          &dtm, NULL) != 0)
+                          &dtm, NULL, rec) != 0)
             return -1.0;
         struct Copy { void *dst; const void *src; size_t n; };
         const long long nts = dims < MAX_NCELLS ? dims : MAX_NCELLS;

@@ -119,7 +119,7 @@ long long dengo_server_submit(dengo_server *srv, int network, long long ncells, 
  * copied (no arena space is used), so a patch's state can stay on the device
  * between its successive requests.  The arrays must stay valid and untouched
  * until the request completes; its completion record carries the status
- * counts.  Each pointer is checked to be device memory that holds the patch
+ * counts when d_status is given.  Each pointer is checked to be device memory that holds the patch
  * (hipPointerGetAttributes, hipMemGetAddressRange).  With cfg->dense_order, a
  * dense-network patch's cells are handed out densest first as for host
  * patches; the order is sorted on the device in arena scratch (4 x 8 bytes per
