@@ -1,6 +1,8 @@
-"""The autotuner's accuracy measure on the CPU (torch tensors)."""
+"""The autotuner's accuracy measure, cache key and precision rule on the CPU
+(torch tensors)."""
 import torch
 
+from dengo_amd import autotune
 from dengo_amd.autotune import agreement
 
 
@@ -35,3 +37,20 @@ def test_cache_digest_covers_tables_and_tolerances():
     assert d != _digest(s1, rt * 10, at, None)            # other rtol
     assert d != _digest(s1, rt, at * 10, None)            # other atol scale
     assert d != _digest(SimpleNamespace(tables=tab, T_bounds=(1e4, 1e9)), rt, at, None)
+
+
+def test_choice_prefers_accuracy_within_the_margin():
+    def t(ms, ok=True):
+        return {"kernel_ms": ms, "accepted": ok}
+    # a near-tie goes to the more accurate sweep
+    assert autotune.choose_precision({"fp64": t(3000), "mixed": t(1407),
+                                      "fp64-fast": t(1424)}) == "fp64-fast"
+    # a clear gain goes to the faster one
+    assert autotune.choose_precision({"fp64": t(233), "mixed": t(33),
+                                      "fp64-fast": t(82)}) == "mixed"
+    # rejected variants never win, however fast
+    assert autotune.choose_precision({"fp64": t(100), "mixed": t(10, False),
+                                      "fp64-fast": t(99)}) == "fp64"
+    assert autotune.choose_precision({"fp64": t(100), "fp64-fast": t(96)}) == "fp64"
+    assert autotune.choose_precision({"fp64": t(100), "fp64-fast": t(96)}, margin=0.0) == \
+        "fp64-fast"
