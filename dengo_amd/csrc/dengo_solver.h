@@ -27,6 +27,7 @@
 
 #include "be_driver.h"
 #include "be_fast.h"
+#include "prim_lean.h"
 #include "ma_net.h"
 #include "gen_net.h"
 #include "dengo_h5.h"
@@ -920,8 +921,19 @@ struct Host {
     template <int MODE>
     static void kernels(void (**kp)(KParams<Net>), void (**kc)(KParams<Net>))
     {
-        *kp = k_chem<Net, true, MODE>;
-        *kc = k_chem<Net, false, MODE>;
+// The primordial FP64-fast sweep runs as the register-lean kernel
+// (prim_lean.h, two waves per SIMD); building with -DDENGO_LEAN_PRIMORDIAL=0
+// selects the register-resident k_chem form, whose results are bit-identical.
+#ifndef DENGO_LEAN_PRIMORDIAL
+#define DENGO_LEAN_PRIMORDIAL 1
+#endif
+        if constexpr (DENGO_LEAN_PRIMORDIAL && MODE == kFast64 && std::is_same<Net, PrimNet>::value) {
+            *kp = k_chem_lean<true>;
+            *kc = k_chem_lean<false>;
+        } else {
+            *kp = k_chem<Net, true, MODE>;
+            *kc = k_chem<Net, false, MODE>;
+        }
     }
 
     // Workgroups of a persistent launch that fill the GPU at full occupancy.
