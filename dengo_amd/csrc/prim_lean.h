@@ -20,7 +20,9 @@
 //   rates are never live together;
 // * the Newton right-hand side rides along the elimination as an extra
 //   column (forward substitution inside the factorization), and each
-//   reciprocal pivot takes the register of the diagonal entry it replaces.
+//   reciprocal pivot takes the register of the diagonal entry it replaces;
+// * a candidate row's exchange (a select per entry of both rows) runs only if
+//   some lane of the wave chose that row; the pivots are unchanged.
 //
 // Reference semantics: BE_chem_solve (dengo/solvers/BE_chem_solve.C:79-241)
 // and the substep loop of dengo_evolve
@@ -205,17 +207,19 @@ DENGO_D int sweep(LeanCell &c, const KParams<PrimNet> &p)
                     constexpr int i = CV(ii);
                     if constexpr (kPlan.cand[k][i]) {
                         const bool sw = (p == i);
-                        static_for<k, M>([&](auto jj) {
-                            constexpr int j = CV(jj);
-                            if constexpr (kPlan.S[k][k][j] || kPlan.S[k][i][j]) {
-                                const double a = A[k][j], e = A[i][j];
-                                A[k][j] = sw ? e : a;
-                                A[i][j] = sw ? a : e;
-                            }
-                        });
-                        const double ya = y[k], ye = y[i];
-                        y[k] = sw ? ye : ya;
-                        y[i] = sw ? ya : ye;
+                        if (__any(sw)) {
+                            static_for<k, M>([&](auto jj) {
+                                constexpr int j = CV(jj);
+                                if constexpr (kPlan.S[k][k][j] || kPlan.S[k][i][j]) {
+                                    const double a = A[k][j], e = A[i][j];
+                                    A[k][j] = sw ? e : a;
+                                    A[i][j] = sw ? a : e;
+                                }
+                            });
+                            const double ya = y[k], ye = y[i];
+                            y[k] = sw ? ye : ya;
+                            y[i] = sw ? ya : ye;
+                        }
                     }
                 });
             }
