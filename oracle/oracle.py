@@ -155,11 +155,14 @@ def prim_evolve(inp, rtol, atol, tables, dtf, dt=-1.0, z=-1.0, T_bounds=(1.0, 1e
         be = ctypes.cast(ref.ref_BE_chem_solve, ctypes.c_void_p)
     tab = np.ascontiguousarray(tables, dtype=np.float64)
     dtc = ctypes.c_double(dt)
-    ttot = load().orc_prim_evolve(
-        dtf, ctypes.byref(dtc), z, _ptr(out), _ptr(rtol_c), _ptr(atol_o), n, _ptr(tab),
-        tab.shape[1], T_bounds[0], T_bounds[1], int(input_is_number), _ptr(status, _ip),
-        _ptr(nsub, _ip), _ptr(nrej, _ip), _ptr(nsweep, _ip), _ptr(dt_out), _ptr(T_out),
-        be, nthreads, flags)
+    try:
+        ttot = load().orc_prim_evolve(
+            dtf, ctypes.byref(dtc), z, _ptr(out), _ptr(rtol_c), _ptr(atol_o), n, _ptr(tab),
+            tab.shape[1], T_bounds[0], T_bounds[1], int(input_is_number), _ptr(status, _ip),
+            _ptr(nsub, _ip), _ptr(nrej, _ip), _ptr(nsweep, _ip), _ptr(dt_out), _ptr(T_out),
+            be, nthreads, flags)
+    finally:
+        load().orc_set_max_substeps(50000)
     return dict(state=out, atol=atol_o, ttot=ttot, dt=dtc.value, status=status,
                 nsub=nsub, nrej=nrej, nsweep=nsweep, dt_cell=dt_out, T=T_out)
 
@@ -323,11 +326,14 @@ def ma_evolve(net, inp, rtol, atol, tables, dtf, T_bounds, dt=-1.0, z=-1.0,
     dt_out, T_out = np.zeros(n), np.zeros(n)
     tab = np.ascontiguousarray(tables, dtype=np.float64)
     dtc = ctypes.c_double(dt)
-    ttot = lib.orc_ma_evolve(
-        ctypes.byref(net), dtf, ctypes.byref(dtc), z, _ptr(out), _ptr(rtol_c), _ptr(atol_o), n,
-        _ptr(tab), tab.shape[1], T_bounds[0], T_bounds[1], int(input_is_number),
-        _ptr(status, _ip), _ptr(nsub, _ip), _ptr(nrej, _ip), _ptr(nsweep, _ip), _ptr(dt_out),
-        _ptr(T_out), None, nthreads, flags)
+    try:
+        ttot = lib.orc_ma_evolve(
+            ctypes.byref(net), dtf, ctypes.byref(dtc), z, _ptr(out), _ptr(rtol_c), _ptr(atol_o),
+            n, _ptr(tab), tab.shape[1], T_bounds[0], T_bounds[1], int(input_is_number),
+            _ptr(status, _ip), _ptr(nsub, _ip), _ptr(nrej, _ip), _ptr(nsweep, _ip),
+            _ptr(dt_out), _ptr(T_out), None, nthreads, flags)
+    finally:
+        lib.orc_set_max_substeps(50000)
     return dict(state=out, atol=atol_o, ttot=ttot, dt=dtc.value, status=status, nsub=nsub,
                 nrej=nrej, nsweep=nsweep, dt_cell=dt_out, T=T_out)
 
@@ -359,7 +365,7 @@ def big_rates(net, T):
     return out
 
 
-def big_evolve(net, x, T, rtol, atol, dtf, dt=-1.0, nthreads=0, max_substeps=None):
+def big_evolve(net, x, T, rtol, atol, dtf, dt=-1.0, nthreads=0, max_substeps=50000):
     """Isothermal BE integration of the synthetic network, per cell.  Returns
     (ttot, dt, outputs dict) and updates x (number densities) in place."""
     lib = load()
@@ -375,16 +381,16 @@ def big_evolve(net, x, T, rtol, atol, dtf, dt=-1.0, nthreads=0, max_substeps=Non
     out = {k: np.zeros(ncells, dtype=np.int32) for k in ("status", "nsub", "nrej", "nsweep")}
     out["dt"] = np.zeros(ncells)
     dtv = ctypes.c_double(dt)
-    if max_substeps is not None:
-        lib.orc_set_max_substeps(int(max_substeps))
+    # the cap is a library-wide setting: every evolve call sets its own and
+    # restores the reference's 50000 on the way out
+    lib.orc_set_max_substeps(int(max_substeps))
     try:
         ttot = lib.orc_big_evolve(ctypes.byref(bn), dtf, ctypes.byref(dtv), _ptr(T), _ptr(x),
                                   _ptr(rt), _ptr(at), ncells, _ptr(out["status"], _ip),
                                   _ptr(out["nsub"], _ip), _ptr(out["nrej"], _ip),
                                   _ptr(out["nsweep"], _ip), _ptr(out["dt"]), nthreads)
     finally:
-        if max_substeps is not None:
-            lib.orc_set_max_substeps(50000)
+        lib.orc_set_max_substeps(50000)
     return ttot, dtv.value, out
 
 

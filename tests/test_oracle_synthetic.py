@@ -155,3 +155,19 @@ def test_fixture_matches_fresh_oracle_run():
     assert np.array_equal(x2, d["input"]) and np.array_equal(T2, d["T"])
     rtol, atol = tolerances(x2)
     assert np.array_equal(rtol, d["rtol"]) and np.array_equal(atol, d["atol"])
+
+
+def test_substep_cap_does_not_leak_between_calls():
+    # the oracle's substep cap is a library-wide setting: a call with a small
+    # cap must not cap the next call that asks for the reference's default
+    from dengo_amd.ics import primordial_ics, tolerances
+    from dengo_amd.networks import primordial_tables
+    _, tab = primordial_tables()
+    inp, _, _ = primordial_ics(2, seed=5)
+    rt, at = tolerances(inp)
+    r = oracle.prim_evolve(inp, rt, at, tab, 1e12, max_substeps=3)
+    assert (r["nsub"] <= 3).all()
+    tn = TinyNet(2, [[0, 1, 1, 1]], [0.0])
+    x = np.array([[2.0, 3.0]])
+    _, _, out = oracle.big_evolve(tn, x, np.array([300.0]), np.full_like(x, 1e-9), 1e-9 * x, 1e12)
+    assert out["status"][0] == 0 and out["nsub"][0] == 97
