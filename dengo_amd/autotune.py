@@ -14,8 +14,14 @@ workload:
    time is within `margin` (default 5%) of the fastest wins, so that a
    timing difference inside the run-to-run noise does not trade accuracy for
    nothing.
-2. schedule: persistent (full grid, half grid) or one cell per lane, timed at
-   the chosen precision; the fastest wins.
+2. schedule: one cell per lane, or persistent with a number of resident
+   workgroups (the cells in flight): the full-occupancy grid, or 4, 2 or 1
+   workgroups per CU (lane-per-cell networks: 64 cells per workgroup; the
+   dense network: one cell per 512-thread workgroup, full grid or one per two
+   CUs), timed at the chosen precision; the fastest wins.  Fewer cells in
+   flight shorten the tail of a batch whose cost sits in a few long cells:
+   each lane then works through more cells, and a lone wave on a SIMD runs its
+   cells' sweeps faster.
 
 Timings are kernel times from HIP events on the launch stream.  The sample
 should be as large as the batches the choice is for: the variants' ranking
@@ -46,6 +52,22 @@ def choose_precision(trials, margin=0.05):
     fastest = min(ok.values())
     near = [k for k, ms in ok.items() if ms <= fastest * (1.0 + margin)]
     return min(near, key=ACCURACY_ORDER.index)
+
+
+def schedule_candidates(cu_count, dense):
+    """(name, options) of the schedules the tuner times: the persistent
+    schedule at full occupancy (grid 0: the library's own count of resident
+    workgroups, which depends on the kernel's registers and LDS), at a number
+    of workgroups per CU below it, and one cell per lane.  A grid above the
+    resident count runs like the full grid (the extra workgroups find the
+    queue empty), so the per-CU counts need not know the kernel's occupancy."""
+    per_cu = (0.5,) if dense else (4, 2, 1)
+    cands = [("persistent", dict(schedule=runtime.PERSISTENT, grid=0))]
+    for k in per_cu:
+        cands.append(("persistent_%g_per_cu" % k,
+                      dict(schedule=runtime.PERSISTENT, grid=max(1, int(cu_count * k)))))
+    cands.append(("per_cell", dict(schedule=runtime.PER_CELL)))
+    return cands
 
 
 def _name(solver):
@@ -142,13 +164,9 @@ def tune(solver, x, rtol, atol, dtf, T=None, sample=4096, rtol_bar=1e-6, min_agr
             log("autotune %s %s: %.1f ms, status %.4f, within %.4f%s" % (
                 _name(solver), PRECISION_NAMES[prec], ms, sa, wb, "" if ok else " (rejected)"))
     best_prec = {v: k for k, v in PRECISION_NAMES.items()}[choose_precision(trials, margin)]
-    # resident workgroups that fill the GPU: one dense cell per CU, one
-    # lane-per-cell wave per SIMD
-    full = dev.multi_processor_count * (1 if isinstance(solver, runtime.SyntheticSolver) else 4)
     sched = {}
-    cands = [("persistent", dict(schedule=runtime.PERSISTENT, grid=0)),
-             ("persistent_half_grid", dict(schedule=runtime.PERSISTENT, grid=max(1, full // 2))),
-             ("per_cell", dict(schedule=runtime.PER_CELL))]
+    cands = schedule_candidates(dev.multi_processor_count,
+                                isinstance(solver, runtime.SyntheticSolver))
     best = None
     for name, opt in cands:
         ms, _, _ = timed(precision=best_prec, **opt)

@@ -6,13 +6,14 @@
 // a lane needs at most 256 registers (VGPRs + AGPRs) and two waves share each
 // SIMD, covering each other's dependency and memory stalls:
 //
-// * the values read once or twice per sweep -- u0 (state at the start of the
-//   BE call), the scaling sc and its inverse, the scaled atol -- sit in LDS,
+// * the values read once or twice per sweep -- the state before the substep
+//   (prev), the scaling sc and its inverse, the scaled atol -- sit in LDS,
 //   40 doubles per lane (20 KiB per wave, 160 KiB per CU at 8 waves);
+// * u0, the state at the start of the BE call, is not stored: it is prev
+//   after a rejected substep or at the cell's start, and prev floored at
+//   1e-25 after an accepted one, so one flag per cell recovers it exactly;
 // * rtol is read from the caller's rtol row (one batch of loads, before the
-//   convergence test), and the state before the substep (prev) is kept in
-//   the cell's own input row, which the kernel owns between loading the cell
-//   and storing its result;
+//   convergence test);
 // * each reaction's rate is interpolated and used at once for its flux and
 //   its Jacobian partials; the cooling rates, which only the energy's Newton
 //   step needs (the energy column of the BE matrix is zero), are interpolated
@@ -36,12 +37,14 @@ namespace lean {
 constexpr int N = prim::N, M = prim::M, GE = prim::GE, DE = prim::DE, NTAB = prim::NTAB;
 constexpr int kLanes = 64;
 // LDS slots per lane, each a [slot][lane] column of the block's array
-enum { L_U0 = 0, L_SC = N, L_INV = 2 * N, L_ATOL = 3 * N, kSlots = 4 * N };
+enum { L_PREV = 0, L_SC = N, L_INV = 2 * N, L_ATOL = 3 * N, kSlots = 4 * N };
+enum { kFloored = 1 };
 
 struct LeanCell {
     double u[N];
     double dt, ttot, T;
     int siter, nrej, nsweep, sweep, status;
+    int flags;   // kFloored: u0 is prev floored at 1e-25 (the last substep was accepted)
     long long id;
     double *cold;   // this lane's LDS column: cold[slot * kLanes]
     DENGO_D double &at(int slot) const { return cold[slot * kLanes]; }
@@ -51,7 +54,7 @@ struct LeanCell {
 DENGO_D void begin_cell(LeanCell &c, long long id, const KParams<PrimNet> &p)
 {
     c.id = id;
-    double *in = p.input + id * N;
+    const double *in = p.input + id * N;
     const double *atr = p.atol + id * N;
     double at[N];
 #pragma unroll
@@ -59,6 +62,7 @@ DENGO_D void begin_cell(LeanCell &c, long long id, const KParams<PrimNet> &p)
         c.u[k] = in[k];
         at[k] = atr[k];
     }
+    c.flags = 0;
     if (!p.input_is_number) {
 #pragma unroll
         for (int k = 0; k < N; ++k) {
@@ -85,18 +89,25 @@ DENGO_D void begin_cell(LeanCell &c, long long id, const KParams<PrimNet> &p)
         c.at(L_SC + k) = c.u[k];
         c.at(L_INV + k) = 1.0 / c.u[k];
         c.at(L_ATOL + k) = at[k];
-        in[k] = c.u[k];   // prev
+        c.at(L_PREV + k) = c.u[k];
     }
 }
 
-// BE_chem_solve entry (be_fast.h begin_be_fast): remember u0, scale atol.
+// BE_chem_solve entry (be_fast.h begin_be_fast): scale atol (u0 is the
+// current u, which u0() recovers from prev).
 DENGO_D void begin_be(LeanCell &c)
 {
 #pragma unroll
-    for (int i = 0; i < N; ++i) c.at(L_U0 + i) = c.u[i];
-#pragma unroll
     for (int i = 0; i < N; ++i) c.at(L_ATOL + i) *= c.at(L_INV + i);
     c.sweep = 0;
+}
+
+// u0[i]: the state at the start of the BE call (end_be sets u from prev, or
+// from prev with the substep loop's 1e-25 floor).
+DENGO_D double u0(const LeanCell &c, int i)
+{
+    const double v = c.at(L_PREV + i);
+    return ((c.flags & kFloored) && v < 1e-25) ? 1e-25 : v;
 }
 
 // One Newton sweep (be_fast.h sweep_fast<PrimNet, double>, same values).
@@ -174,7 +185,7 @@ DENGO_D int sweep(LeanCell &c, const KParams<PrimNet> &p)
             constexpr int i = CV(ii);
             const double inv = c.at(L_INV + i);
             const double dtinv = dt * inv;
-            y[i] = (u[i] - c.at(L_U0 + i)) * inv - dtinv * f[i];
+            y[i] = (u[i] - u0(c, i)) * inv - dtinv * f[i];
             static_for<0, M>([&](auto jj) {
                 constexpr int j = CV(jj);
                 if constexpr (i == j) A[i][j] = 1.0 - dtinv * A[i][j];
@@ -277,7 +288,7 @@ DENGO_D int sweep(LeanCell &c, const KParams<PrimNet> &p)
         const double inv = c.at(L_INV + GE);
         const double dtinv = c.dt * inv;
         const double w = -dtinv * rmd;
-        yge = (u[GE] - c.at(L_U0 + GE)) * inv - dtinv * fge;
+        yge = (u[GE] - u0(c, GE)) * inv - dtinv * fge;
 #pragma unroll
         for (int j = 0; j < M; ++j) yge = yge - (w * g[j] * c.at(L_SC + j)) * y[j];
     }
@@ -309,7 +320,6 @@ DENGO_D int sweep(LeanCell &c, const KParams<PrimNet> &p)
 // BE exit and one pass of the substep loop (be_fast.h end_be_fast).
 DENGO_D bool end_be(LeanCell &c, int code, const KParams<PrimNet> &p)
 {
-    double *prev = p.input + c.id * N;
     int rv = code == SW_CONVERGED ? 0 : 1;
 #pragma unroll
     for (int i = 0; i < N; ++i) c.at(L_ATOL + i) *= c.at(L_SC + i);
@@ -325,15 +335,17 @@ DENGO_D bool end_be(LeanCell &c, int code, const KParams<PrimNet> &p)
         c.ttot += c.dt;
         c.dt = (c.dt * 1.1 < p.dtf - c.ttot) ? c.dt * 1.1 : p.dtf - c.ttot;
 #pragma unroll
-        for (int i = 0; i < N; ++i) prev[i] = c.u[i];
+        for (int i = 0; i < N; ++i) c.at(L_PREV + i) = c.u[i];
 #pragma unroll
         for (int i = 0; i < N; ++i)
             if (c.u[i] < 1e-25) c.u[i] = 1e-25;
+        c.flags |= kFloored;
     } else {
         c.nrej++;
         c.dt /= 2.0;
 #pragma unroll
-        for (int i = 0; i < N; ++i) c.u[i] = prev[i];
+        for (int i = 0; i < N; ++i) c.u[i] = c.at(L_PREV + i);
+        c.flags &= ~kFloored;
         if (c.dt / p.dtf < 1e-15) {
             c.status = 1;
             return true;

@@ -54,3 +54,14 @@ def test_choice_prefers_accuracy_within_the_margin():
     assert autotune.choose_precision({"fp64": t(100), "fp64-fast": t(96)}) == "fp64"
     assert autotune.choose_precision({"fp64": t(100), "fp64-fast": t(96)}, margin=0.0) == \
         "fp64-fast"
+
+
+def test_schedule_candidates():
+    from dengo_amd import runtime
+    c = dict(autotune.schedule_candidates(256, dense=False))
+    assert c["persistent"] == dict(schedule=runtime.PERSISTENT, grid=0)
+    assert [c["persistent_%d_per_cu" % k]["grid"] for k in (4, 2, 1)] == [1024, 512, 256]
+    assert c["per_cell"] == dict(schedule=runtime.PER_CELL)
+    d = dict(autotune.schedule_candidates(256, dense=True))
+    assert sorted(d) == ["per_cell", "persistent", "persistent_0.5_per_cu"]
+    assert d["persistent_0.5_per_cu"]["grid"] == 128
