@@ -12,8 +12,11 @@
 // * u0, the state at the start of the BE call, is not stored: it is prev
 //   after a rejected substep or at the cell's start, and prev floored at
 //   1e-25 after an accepted one, so one flag per cell recovers it exactly;
-// * rtol is read from the caller's rtol row (one batch of loads, before the
-//   convergence test);
+// * rtol is read before the convergence test in one batch of loads, from the
+//   first cell's row when the cell's own row holds the same bits (the usual
+//   case: one rtol for the whole call, as the emitted main sets it), so that
+//   every such lane of a wave loads the same line, which stays in L1; from
+//   the cell's own row otherwise;
 // * each reaction's rate is interpolated and used at once for its flux and
 //   its Jacobian partials; the cooling rates, which only the energy's Newton
 //   step needs (the energy column of the BE matrix is zero), are interpolated
@@ -38,13 +41,15 @@ constexpr int N = prim::N, M = prim::M, GE = prim::GE, DE = prim::DE, NTAB = pri
 constexpr int kLanes = 64;
 // LDS slots per lane, each a [slot][lane] column of the block's array
 enum { L_PREV = 0, L_SC = N, L_INV = 2 * N, L_ATOL = 3 * N, kSlots = 4 * N };
-enum { kFloored = 1 };
+enum { kFloored = 1, kOwnRtol = 2 };
 
 struct LeanCell {
     double u[N];
     double dt, ttot, T;
     int siter, nrej, nsweep, sweep, status;
-    int flags;   // kFloored: u0 is prev floored at 1e-25 (the last substep was accepted)
+    // kFloored: u0 is prev floored at 1e-25 (the last substep was accepted);
+    // kOwnRtol: the cell's rtol row differs from the first cell's
+    int flags;
     long long id;
     double *cold;   // this lane's LDS column: cold[slot * kLanes]
     DENGO_D double &at(int slot) const { return cold[slot * kLanes]; }
@@ -62,7 +67,12 @@ DENGO_D void begin_cell(LeanCell &c, long long id, const KParams<PrimNet> &p)
         c.u[k] = in[k];
         at[k] = atr[k];
     }
-    c.flags = 0;
+    const double *rtr = p.rtol + id * N;
+    bool same = true;
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+        same = same && __double_as_longlong(rtr[k]) == __double_as_longlong(p.rtol[k]);
+    c.flags = same ? 0 : kOwnRtol;
     if (!p.input_is_number) {
 #pragma unroll
         for (int k = 0; k < N; ++k) {
@@ -296,7 +306,7 @@ DENGO_D int sweep(LeanCell &c, const KParams<PrimNet> &p)
     // u -= scale * y; max-norm test in scaled units (first unconverged entry)
     // the whole rtol row first: loads inside the test's short-circuit would
     // each wait for the one before
-    const double *rtrow = p.rtol + c.id * N;
+    const double *rtrow = p.rtol + ((c.flags & kOwnRtol) ? c.id * N : 0);
     double rtol[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) rtol[i] = rtrow[i];

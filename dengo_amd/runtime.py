@@ -464,8 +464,11 @@ class GeneratedSolver(NetworkSolver):
     """A network solver built from ChemicalNetwork.write_solver output:
     library lib<name>_solver.so and the table blob <name>_tables.bin."""
 
-    def __init__(self, lib_path, name, nspecies, tables, T_bounds):
+    def __init__(self, lib_path, name, nspecies, tables, T_bounds, energy_column=None):
         self.N = nspecies
+        # the energy's column in the caller's rows (the last unless the
+        # network's species sort after it)
+        self.energy_column = nspecies - 1 if energy_column is None else energy_column
         NetworkSolver.__init__(self, name, tables, T_bounds,
                                lib=load_solver_library(lib_path, name))
 

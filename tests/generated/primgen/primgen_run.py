@@ -21,7 +21,8 @@ TABLES = os.path.join(HERE, 'primgen_tables.bin')
 def load():
     """The solver with the tables written beside this file."""
     tables, T_bounds, _ = read_blob(TABLES)
-    return runtime.GeneratedSolver(LIBRARY, SOLVER_NAME, len(SPECIES), tables, T_bounds)
+    return runtime.GeneratedSolver(LIBRARY, SOLVER_NAME, len(SPECIES), tables, T_bounds,
+                                   energy_column=SPECIES.index(ENERGY))
 
 
 def run_primgen(ics, tf, niter=10000, intermediate=1, z=-1.0, solver=None):

@@ -65,3 +65,26 @@ def test_schedule_candidates():
     d = dict(autotune.schedule_candidates(256, dense=True))
     assert sorted(d) == ["per_cell", "persistent", "persistent_0.5_per_cu"]
     assert d["persistent_0.5_per_cu"]["grid"] == 128
+
+
+def test_agreement_ignores_trace_species_but_not_the_energy():
+    ref = torch.tensor([[1.0, 1e-25, 5.0]], dtype=torch.float64)   # species, trace species, energy
+    st = torch.zeros(1, dtype=torch.int32)
+    species = torch.tensor([True, True, False])
+    x = ref.clone()
+    x[0, 1] = 3e-25                  # a species at the floor: relative error 2, not counted
+    assert agreement(ref, st, x, st, 1e-6, species) == (1.0, 1.0)
+    assert agreement(ref, st, x, st, 1e-6) == (1.0, 1.0)    # every column a species
+    y = x.clone()
+    y[0, 0] = 1.0 + 2e-6             # a major species is counted
+    assert agreement(ref, st, y, st, 1e-6, species) == (1.0, 0.0)
+    x[0, 2] = 5.0 * (1 + 1e-5)       # the energy is always counted
+    assert agreement(ref, st, x, st, 1e-6, species) == (1.0, 0.0)
+
+
+def test_species_mask():
+    from types import SimpleNamespace
+    m = autotune.species_mask(SimpleNamespace(N=4))
+    assert m.tolist() == [True, True, True, False]
+    m = autotune.species_mask(SimpleNamespace(N=4, energy_column=1))
+    assert m.tolist() == [True, False, True, True]

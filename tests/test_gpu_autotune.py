@@ -56,3 +56,37 @@ def test_primordial_and_oxygen_choices(solver, tmp_path):
     finally:
         oxy.close()
     assert len(json.load(open(cache))) == 2
+
+
+def test_dense_network_choice():
+    """The dense network: precision among FP64 and FP64-fast (it has no FP32
+    factorization), schedule among the full persistent grid, one workgroup per
+    two CUs and one workgroup per cell without the queue; timed on the
+    fixture's cells that run under 5,000 sweeps (the capped one alone would
+    take seconds per trial)."""
+    import os
+
+    import numpy as np
+
+    from dengo_amd.synthetic import SyntheticNetwork
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                             "synthetic_cells.npz"))
+    keep = np.nonzero(g["nsweep"] < 5000)[0]
+    s = runtime.SyntheticSolver(SyntheticNetwork(seed=int(g["net_seed"])))
+    try:
+        x, T, rt, at = _dev(*(np.ascontiguousarray(g[k][keep])
+                              for k in ("input", "T", "rtol", "atol")))
+        print("autotune synthetic, %d cells" % len(keep), flush=True)
+        # min_agree 0.9: in a sample this small one cell whose substep path
+        # the rounding decides (as test_gpu_synthetic allows) is 4% of it
+        r = autotune.tune(s, x, rt, at, float(g["dtf"]), T=T, sample=len(keep), min_agree=0.9)
+        trials = r["precision_trials"]
+        assert set(trials) == {"fp64", "fp64-fast"}
+        assert trials["fp64-fast"]["accepted"], trials
+        assert r["precision"] == autotune.choose_precision(trials)
+        assert set(r["schedule_ms"]) == {"persistent", "persistent_0.5_per_cu", "per_cell"}
+        assert r["schedule"] in r["schedule_ms"]
+        y = x.clone()
+        s.evolve_device(y, T, rt, at.clone(), float(g["dtf"]), **r["options"])
+    finally:
+        s.close()
