@@ -39,9 +39,9 @@ PEAK_FP64 = 78.6e12      # MI355X FP64 (vector and matrix) TFLOP/s
 # HBM bytes per launch of the FP64-fast persistent kernel on the 128^3
 # primordial workload: FETCH_SIZE + WRITE_SIZE from two rocprofv3 --pmc runs
 # of this script (--calib-copy-mib 1024), each scaled by the counter's
-# reading for the 1 GiB device copy: 1,059,786 + 606,655 KiB (means of the
-# two launches of each run).
-HEADLINE_TRAFFIC_BYTES = 1666441 * 1024
+# reading for the 1 GiB device copy: 1,037,038 + 621,080 KiB (means of the
+# two launches of each run; kernel k_chem_lean<true>).
+HEADLINE_TRAFFIC_BYTES = 1658118 * 1024
 PEAK_FP32 = 157.3e12     # MI355X FP32 vector
 
 
