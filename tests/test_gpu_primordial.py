@@ -215,6 +215,37 @@ def test_fp64_fast_vs_oracle(solver, schedule):
 
 
 @pytest.mark.parametrize("schedule", [runtime.PER_CELL, runtime.PERSISTENT])
+def test_fp64_fast_rtol_by_cell_and_species(solver, prim_tables, schedule):
+    """rtol that differs by cell and by species.  The FP64-fast kernel reads
+    the first cell's rtol row for every cell whose row has the same bits and
+    the cell's own row otherwise; both kinds of cell must integrate with their
+    own tolerances: the oracle's status everywhere, and on 90% of the
+    finished cells its substep path and Newton sweep counts (which depend on
+    rtol), every finished cell within 1e-6."""
+    _, tab = prim_tables
+    n = 192
+    inp, _, _ = primordial_ics(n, seed=41)
+    rtol, atol = tolerances(inp)
+    rng = np.random.default_rng(41)
+    own = rng.random(n) < 0.5
+    own[0] = False
+    rtol[own] = 10.0 ** rng.uniform(-11, -5, size=(int(own.sum()), rtol.shape[1]))
+    o = oracle.prim_evolve(inp, rtol, atol, tab, 1e10, flags=oracle.PORTABLE_LOG)
+    # the test can tell: the first cell's tolerances everywhere change the
+    # sweep counts of many cells with their own rows
+    wrong = oracle.prim_evolve(inp, np.broadcast_to(rtol[0], rtol.shape).copy(), atol, tab, 1e10,
+                               flags=oracle.PORTABLE_LOG)
+    assert (wrong["nsweep"][own] != o["nsweep"][own]).mean() > 0.5
+    print("fp64fast, %d cells, %d with their own rtol row" % (n, own.sum()), flush=True)
+    r = solver.evolve(inp, rtol, atol, 1e10, precision=runtime.FP64_FAST, schedule=schedule)
+    assert np.array_equal(r["status"], o["status"])
+    done = o["status"] == 0
+    assert major_rel(r["state"][done], o["state"][done]).max() < 1e-6
+    same = ((r["nsub"] == o["nsub"]) & (r["nrej"] == o["nrej"]) & (r["nsweep"] == o["nsweep"]))
+    assert same[done & own].mean() >= 0.9 and same[done & ~own].mean() >= 0.9
+
+
+@pytest.mark.parametrize("schedule", [runtime.PER_CELL, runtime.PERSISTENT])
 def test_mixed_vs_oracle(solver, schedule):
     """FP32 LU with FP64 iterative refinement and an FP64 fallback.  Cells that
     reach dtf in both agree to 1e-6 relative on the same substep path.  Known
