@@ -681,5 +681,91 @@ def render_gen_network(net, name):
     L += body([jac[k] for k in keys], ["J[%d]" % (c * n + r) for r, c in keys])
     for r, c in zero:
         L.append("        J[%d] = 0.0;" % (c * n + r))
-    L += ["    }", "};", "}  // namespace net_%s" % name, "}  // namespace dengo", ""]
+    L += ["    }", ""]
+    L += _lean_block(net, internal, syms, rates, rhs, jac, body)
+    L += ["};", "}  // namespace net_%s" % name, "}  // namespace dengo", ""]
     return "\n".join(L)
+
+
+def _lean_block(net, internal, syms, rates, rhs, jac, body):
+    """The network's forms for the register-lean FP64-fast sweep
+    (dengo_amd/csrc/lean_sweep.h, adapter LeanGen in gen_net.h): the reaction
+    table of its species rows (rate table, reactant slots, net change per
+    species), the tables its energy row reads, and the energy row alone: the
+    cooling sum and its gradient by species.  The species rows are mass action
+    (dengo/chemical_network.py:176-180, dengo/reaction_classes.py:148-165);
+    the table is checked against them symbolically, and a network whose rows
+    it does not reproduce gets LEAN = false (the register-resident sweep)."""
+    import sympy
+    n = len(internal)
+    ge = n - 1
+    species = [s.name for s in internal]
+    index = {nm: i for i, nm in enumerate(species)}
+    table = {str(r): t for t, r in enumerate(rates)}
+    rxns = list(net.ordered_reactions())
+    rows = []
+    ok = True
+    for r in rxns:
+        slots = []
+        for cnt, s in r.left_side:
+            if s.name not in index or int(cnt) != cnt:
+                ok = False
+                break
+            slots += [index[s.name]] * int(cnt)
+        if not ok:
+            break
+        rows.append((table[str(r.coeff_sym)], slots, [int(r.net_change(nm)) for nm in species]))
+    if ok:
+        for i in range(ge):
+            mass_action = sympy.Integer(0)
+            for t, slots, nu in rows:
+                if nu[i]:
+                    term = rates[t]
+                    for q in slots:
+                        term = term * syms[q]
+                    mass_action += nu[i] * term
+            if sympy.expand(rhs[i] - mass_action) != 0:
+                ok = False
+                break
+    L = ["    // register-lean FP64-fast sweep (lean_sweep.h): reaction table of the species",
+         "    // rows, checked against them symbolically, and the energy row alone"]
+    if not ok or not rows:
+        return L + ["    static constexpr bool LEAN = false;"]
+    S = max(max(len(sl) for _, sl, _ in rows), 1)
+    used = set()
+    grad = [jac.get((ge, c), sympy.Integer(0)) for c in range(ge)]
+    for e in [rhs[ge]] + grad:
+        used |= {a.name for a in e.free_symbols}
+    etab = ["true" if str(r) in used else "false" for r in rates]
+    L += ["    static constexpr bool LEAN = true;",
+          "    static constexpr int NRXN = %d, MAXSLOT = %d;" % (len(rows), S),
+          "    static constexpr int RXN_TAB[NRXN] = {%s};" % ", ".join(str(t) for t, _, _ in rows),
+          "    static constexpr int RXN_NSLOT[NRXN] = {%s};" % ", ".join(
+              str(len(sl)) for _, sl, _ in rows),
+          "    static constexpr int RXN_SLOT[NRXN][MAXSLOT] = {%s};" % ", ".join(
+              "{%s}" % ", ".join(str(x) for x in sl + [0] * (S - len(sl))) for _, sl, _ in rows),
+          "    static constexpr int NU[NRXN][N] = {%s};" % ", ".join(
+              "{%s}" % ", ".join(str(v) for v in nu) for _, _, nu in rows),
+          "    // tables the energy row reads",
+          "    static constexpr bool ETAB[NTAB] = {%s};" % ", ".join(etab),
+          "",
+          "    static DENGO_GEN_FN double cool(const double (&u)[N], const double (&rs)[NTAB], "
+          "double T, double z)",
+          "    {",
+          "        (void)T;",
+          "        (void)z;",
+          "        double e;"]
+    L += body([rhs[ge]], ["e"])
+    L += ["        return e;",
+          "    }",
+          "",
+          "    static DENGO_GEN_FN void cool_grad(const double (&u)[N], const double (&rs)[NTAB], "
+          "double T,",
+          "                                       double z, double (&g)[N])",
+          "    {",
+          "        (void)T;",
+          "        (void)z;",
+          "        g[%d] = 0.0;" % ge]
+    L += body(grad, ["g[%d]" % c for c in range(ge)])
+    L += ["    }"]
+    return L

@@ -27,7 +27,7 @@
 
 #include "be_driver.h"
 #include "be_fast.h"
-#include "prim_lean.h"
+#include "lean_sweep.h"
 #include "ma_net.h"
 #include "gen_net.h"
 #include "dengo_h5.h"
@@ -918,22 +918,49 @@ struct Host {
         free(d);
     }
 
+    // The FP64-fast sweep runs as the register-lean kernel (lean_sweep.h, two
+    // waves per SIMD) for every network with a lean adapter (lean::LeanOf)
+    // whose lean kernel compiled without scratch and fits at least six 64-lane
+    // workgroups per CU; otherwise, and for the other precisions, as the
+    // register-resident k_chem.  The two forms of the primordial network are
+    // bit-identical.  Building with -DDENGO_LEAN=0 selects k_chem everywhere.
+#ifndef DENGO_LEAN
+#define DENGO_LEAN 1
+#endif
+    template <class LT>
+    static bool lean_fits()
+    {
+        static int fits = -1;   // per process: the kernels are the same on every device
+        if (fits < 0) {
+            int ok = 1;
+            for (const void *k : {(const void *)k_chem_lean<LT, true>,
+                                  (const void *)k_chem_lean<LT, false>}) {
+                hipFuncAttributes a;
+                int per_cu = 0;
+                if (hipFuncGetAttributes(&a, k) != hipSuccess || a.localSizeBytes != 0 ||
+                    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kBlock, 0) !=
+                        hipSuccess || per_cu < 6)
+                    ok = 0;
+            }
+            (void)hipGetLastError();
+            fits = ok;
+        }
+        return fits == 1;
+    }
+
     template <int MODE>
     static void kernels(void (**kp)(KParams<Net>), void (**kc)(KParams<Net>))
     {
-// The primordial FP64-fast sweep runs as the register-lean kernel
-// (prim_lean.h, two waves per SIMD); building with -DDENGO_LEAN_PRIMORDIAL=0
-// selects the register-resident k_chem form, whose results are bit-identical.
-#ifndef DENGO_LEAN_PRIMORDIAL
-#define DENGO_LEAN_PRIMORDIAL 1
-#endif
-        if constexpr (DENGO_LEAN_PRIMORDIAL && MODE == kFast64 && std::is_same<Net, PrimNet>::value) {
-            *kp = k_chem_lean<true>;
-            *kc = k_chem_lean<false>;
-        } else {
-            *kp = k_chem<Net, true, MODE>;
-            *kc = k_chem<Net, false, MODE>;
+        using LT = typename lean::LeanOf<Net>::type;
+        if constexpr (DENGO_LEAN && MODE == kFast64 && !std::is_void<LT>::value) {
+            if (lean_fits<LT>()) {
+                *kp = k_chem_lean<LT, true>;
+                *kc = k_chem_lean<LT, false>;
+                return;
+            }
         }
+        *kp = k_chem<Net, true, MODE>;
+        *kc = k_chem<Net, false, MODE>;
     }
 
     // Workgroups of a persistent launch that fill the GPU at full occupancy.

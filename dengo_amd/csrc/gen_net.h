@@ -18,7 +18,10 @@
 // per-call redshift tables (:362-449), the negative-species check and the
 // energy row divided by the mass density (:460-522, :527-599).
 #pragma once
+#include <type_traits>
+
 #include "be_fast.h"
+#include "lean_sweep.h"
 
 // qualifiers of the generated functions (net_<name>.h)
 #define DENGO_GEN_FN __device__ __forceinline__
@@ -203,5 +206,103 @@ struct FastTraits<GenNet<G>> {
         Net::jac_fast(u, rs, T, rmd, sc, dtinv, env, A, age);
     }
 };
+
+namespace lean {
+// The register-lean FP64-fast sweep of a generated network (lean_sweep.h):
+// the reaction table of its species rows and its energy row, printed by
+// dengo_amd.codegen (G::LEAN); the temperature, interpolation and mass density
+// of GenNet<G>::rhs_fast.  The pivot plan follows the table's structural
+// nonzeros.
+template <class G>
+struct LeanGen {
+    using Net = GenNet<G>;
+    static constexpr int N = G::N, M = N - 1, GE = G::GE, DE = G::DE;
+    static constexpr int NTAB = G::NTAB, NRXN = G::NRXN;
+    static constexpr int table(int r) { return G::RXN_TAB[r]; }
+    static constexpr int nslot(int r) { return G::RXN_NSLOT[r]; }
+    static constexpr int slot(int r, int q) { return G::RXN_SLOT[r][q]; }
+    static constexpr int nu(int r, int s) { return G::NU[r][s]; }
+    struct NZ {
+        constexpr bool operator()(int i, int j) const
+        {
+            for (int r = 0; r < NRXN; ++r) {
+                bool in = false;
+                for (int q = 0; q < nslot(r); ++q) in = in || slot(r, q) == j;
+                if (in && nu(r, i) != 0) return true;
+            }
+            return false;
+        }
+    };
+    static constexpr LUPlan<M> plan = make_plan<M>(NZ{});
+    static DENGO_D double weight(int k) { return G::W[k]; }
+    static DENGO_HD double free_e(int k) { return G::FREE_E[k]; }
+    static constexpr int ext(int k) { return G::EXT[k]; }
+
+    static DENGO_D void prologue(const double (&u)[N], const prim::Env &env, double &T,
+                                 const double2 *&row, double &Tdef, bool &ok, double &rmd)
+    {
+#pragma clang fp contract(fast)
+        double density = 0.0, gsum = 0.0, m = 0.0;
+        static_for<0, N>([&](auto s) {
+            if constexpr (CV(s) != GE && CV(s) != DE) {
+                density += G::W[CV(s)] * u[CV(s)];
+                m += u[CV(s)];
+            }
+            if constexpr (CV(s) != GE) gsum += u[CV(s)] * (G::H2G[CV(s)] ? 2.5 : 1.5);
+        });
+        double t = density * u[GE] * (1.67e-24 / 1.3806504e-16) / gsum;
+        t = t < env.tb.T0 ? env.tb.T0 : (t > env.tb.T1 ? env.tb.T1 : t);
+        T = t;
+        const double logT = plog(t);
+        const int bin_id = (int)(env.tb.idbin * (logT - env.tb.lb));
+        const int b = bin_id <= 0 ? 0 : (bin_id >= env.tb.nbins ? env.tb.nbins - 1 : bin_id);
+        Tdef = (logT - (env.tb.lb + bin_id * env.tb.dbin)) * env.tb.idbin;
+        row = env.tb.pairs + (size_t)b * NTAB;
+        ok = true;
+#pragma unroll
+        for (int s = 0; s < N; ++s) ok = ok && !(u[s] < 0.0);
+        rmd = 1.0 / (m * 1.67e-24);
+    }
+
+    template <int K>
+    static DENGO_D double rate(const double2 *row, double Tdef, const prim::Env &env)
+    {
+#pragma clang fp contract(fast)
+        if constexpr (G::ZTAB[K] >= 0) {
+            return env.zr[G::ZTAB[K]];
+        } else {
+            const double2 v = row[K];
+            return v.x + Tdef * (v.y - v.x);
+        }
+    }
+
+    template <class Rate>
+    static DENGO_D double energy(const double (&u)[N], Rate rate, double T, const prim::Env &env,
+                                 double (&g)[N])
+    {
+        double rs[NTAB];
+        static_for<0, NTAB>([&](auto k) {
+            if constexpr (G::ETAB[CV(k)]) rs[CV(k)] = rate(k);
+            else rs[CV(k)] = 0.0;
+        });
+        G::cool_grad(u, rs, T, env.z, g);
+        return G::cool(u, rs, T, env.z);
+    }
+};
+
+// The lean adapter of a network (void: none)
+template <class Net, class = void>
+struct LeanOf {
+    using type = void;
+};
+template <>
+struct LeanOf<PrimNet> {
+    using type = LeanPrim;
+};
+template <class G>
+struct LeanOf<GenNet<G>, std::enable_if_t<G::LEAN>> {
+    using type = LeanGen<G>;
+};
+}  // namespace lean
 
 }  // namespace dengo

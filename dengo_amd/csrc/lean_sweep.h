@@ -1,14 +1,17 @@
-// Register-lean FP64-fast sweep of the primordial network, for two waves per
-// SIMD.
+// Register-lean FP64-fast sweep, for two waves per SIMD.
 //
-// The arithmetic is that of sweep_fast<PrimNet, double> (be_fast.h), term for
-// term; what changes is where a cell's values live while a sweep runs, so that
-// a lane needs at most 256 registers (VGPRs + AGPRs) and two waves share each
-// SIMD, covering each other's dependency and memory stalls:
+// The arithmetic is that of sweep_fast<Net, double> (be_fast.h): the energy
+// column of the BE matrix is structurally zero, so the species block is
+// factored alone with the compile-time pivot plan, and the energy's Newton
+// step follows from its row.  What changes is where a cell's values live
+// while a sweep runs, so that a lane needs at most 256 registers (VGPRs and
+// AGPRs together) and two waves share each SIMD, covering each other's
+// dependency and memory stalls:
 //
 // * the values read once or twice per sweep -- the state before the substep
 //   (prev), the scaling sc and its inverse, the scaled atol -- sit in LDS,
-//   40 doubles per lane (20 KiB per wave, 160 KiB per CU at 8 waves);
+//   4 N doubles per lane (20 KiB per 64-lane workgroup for N = 10, so eight
+//   workgroups share a CU's 160 KiB);
 // * u0, the state at the start of the BE call, is not stored: it is prev
 //   after a rejected substep or at the cell's start, and prev floored at
 //   1e-25 after an accepted one, so one flag per cell recovers it exactly;
@@ -17,16 +20,24 @@
 //   case: one rtol for the whole call, as the emitted main sets it), so that
 //   every such lane of a wave loads the same line, which stays in L1; from
 //   the cell's own row otherwise;
-// * each reaction's rate is interpolated and used at once for its flux and
-//   its Jacobian partials; the cooling rates, which only the energy's Newton
-//   step needs (the energy column of the BE matrix is zero), are interpolated
-//   after the species solve, when the factor's registers are free; so the 45
-//   rates are never live together;
+// * the species rows are mass action (dengo/chemical_network.py:176-180,
+//   dengo/reaction_classes.py:148-165): each reaction's rate is interpolated
+//   and used at once for its flux and its Jacobian partials; the energy row
+//   (the cooling sum and its gradient), which only the energy's Newton step
+//   needs, is evaluated after the species solve, when the factor's registers
+//   are free; so the rates are never live together;
 // * the Newton right-hand side rides along the elimination as an extra
 //   column (forward substitution inside the factorization), and each
 //   reciprocal pivot takes the register of the diagonal entry it replaces;
 // * a candidate row's exchange (a select per entry of both rows) runs only if
 //   some lane of the wave chose that row; the pivots are unchanged.
+//
+// A network enters through an adapter L (LeanPrim below; LeanGen<G> in
+// gen_net.h): its sizes, pivot plan, caller order and species weights; the
+// temperature, table row and 1/mass density of a state (prologue); a table's
+// interpolated value (rate<K>); its reactions (table, reactant slots, net
+// change per species); and its energy row (energy: the cooling sum, and its
+// gradient by species into g).
 //
 // Reference semantics: BE_chem_solve (dengo/solvers/BE_chem_solve.C:79-241)
 // and the substep loop of dengo_evolve
@@ -37,13 +48,14 @@
 namespace dengo {
 namespace lean {
 
-constexpr int N = prim::N, M = prim::M, GE = prim::GE, DE = prim::DE, NTAB = prim::NTAB;
 constexpr int kLanes = 64;
-// LDS slots per lane, each a [slot][lane] column of the block's array
-enum { L_PREV = 0, L_SC = N, L_INV = 2 * N, L_ATOL = 3 * N, kSlots = 4 * N };
 enum { kFloored = 1, kOwnRtol = 2 };
 
+template <class L>
 struct LeanCell {
+    static constexpr int N = L::N;
+    // LDS slots per lane, each a [slot][lane] column of the block's array
+    static constexpr int L_PREV = 0, L_SC = N, L_INV = 2 * N, L_ATOL = 3 * N, kSlots = 4 * N;
     double u[N];
     double dt, ttot, T;
     int siter, nrej, nsweep, sweep, status;
@@ -56,16 +68,19 @@ struct LeanCell {
 };
 
 // dengo_evolve entry for one cell (be_driver.h begin_cell, same arithmetic).
-DENGO_D void begin_cell(LeanCell &c, long long id, const KParams<PrimNet> &p)
+template <class L>
+DENGO_D void begin_cell(LeanCell<L> &c, long long id, const KParams<typename L::Net> &p)
 {
+    constexpr int N = L::N, GE = L::GE, DE = L::DE;
+    using C = LeanCell<L>;
     c.id = id;
     const double *in = p.input + id * N;
     const double *atr = p.atol + id * N;
     double at[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        c.u[k] = in[k];
-        at[k] = atr[k];
+        c.u[k] = in[L::ext(k)];
+        at[k] = atr[L::ext(k)];
     }
     const double *rtr = p.rtol + id * N;
     bool same = true;
@@ -77,14 +92,14 @@ DENGO_D void begin_cell(LeanCell &c, long long id, const KParams<PrimNet> &p)
 #pragma unroll
         for (int k = 0; k < N; ++k) {
             if (k == GE) continue;
-            c.u[k] /= PrimNet::weight(k) * 1.67e-24;
-            at[k] /= PrimNet::weight(k) * 1.67e-24;
+            c.u[k] /= L::weight(k) * 1.67e-24;
+            at[k] /= L::weight(k) * 1.67e-24;
         }
     }
     double total_e = 0.0;
 #pragma unroll
     for (int k = 0; k < N; ++k)
-        if (k != GE && k != DE) total_e += c.u[k] * PrimNet::free_e(k);
+        if (k != GE && k != DE) total_e += c.u[k] * L::free_e(k);
     c.u[DE] = total_e;
     c.dt = p.dt_in;
     if (c.dt < 0) c.dt = p.dtf / 1e5;
@@ -96,62 +111,49 @@ DENGO_D void begin_cell(LeanCell &c, long long id, const KParams<PrimNet> &p)
     c.status = 0;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        c.at(L_SC + k) = c.u[k];
-        c.at(L_INV + k) = 1.0 / c.u[k];
-        c.at(L_ATOL + k) = at[k];
-        c.at(L_PREV + k) = c.u[k];
+        c.at(C::L_SC + k) = c.u[k];
+        c.at(C::L_INV + k) = 1.0 / c.u[k];
+        c.at(C::L_ATOL + k) = at[k];
+        c.at(C::L_PREV + k) = c.u[k];
     }
 }
 
 // BE_chem_solve entry (be_fast.h begin_be_fast): scale atol (u0 is the
 // current u, which u0() recovers from prev).
-DENGO_D void begin_be(LeanCell &c)
+template <class L>
+DENGO_D void begin_be(LeanCell<L> &c)
 {
+    using C = LeanCell<L>;
 #pragma unroll
-    for (int i = 0; i < N; ++i) c.at(L_ATOL + i) *= c.at(L_INV + i);
+    for (int i = 0; i < L::N; ++i) c.at(C::L_ATOL + i) *= c.at(C::L_INV + i);
     c.sweep = 0;
 }
 
 // u0[i]: the state at the start of the BE call (end_be sets u from prev, or
 // from prev with the substep loop's 1e-25 floor).
-DENGO_D double u0(const LeanCell &c, int i)
+template <class L>
+DENGO_D double u0(const LeanCell<L> &c, int i)
 {
-    const double v = c.at(L_PREV + i);
+    const double v = c.at(LeanCell<L>::L_PREV + i);
     return ((c.flags & kFloored) && v < 1e-25) ? 1e-25 : v;
 }
 
-// One Newton sweep (be_fast.h sweep_fast<PrimNet, double>, same values).
-DENGO_D int sweep(LeanCell &c, const KParams<PrimNet> &p)
+// One Newton sweep (be_fast.h sweep_fast<Net, double>, same equations).
+template <class L>
+DENGO_D int sweep(LeanCell<L> &c, const KParams<typename L::Net> &p)
 {
 #pragma clang fp contract(fast)
-    using namespace prim;
-    const Env &env = p.env;
+    constexpr int N = L::N, M = L::M, GE = L::GE;
+    using C = LeanCell<L>;
+    const auto &env = p.env;
     const double(&u)[N] = c.u;
     c.nsweep++;
-    // temperature and table bin (prim::rhs_fast)
-    const double density = 2.0 * (u[H2I] + u[H2II]) + 1.00794 * (u[HI] + u[HII] + u[HM])
-                         + 4.002602 * (u[HeI] + u[HeII] + u[HeIII]);
-    const double gsum = 2.5 * (u[H2I] + u[H2II])
-                      + 1.5 * (u[HI] + u[HII] + u[HM] + u[HeI] + u[HeII] + u[HeIII] + u[DE]);
-    double t = density * u[GE] * (1.67e-24 / 1.3806504e-16) / gsum;
-    t = t < env.tb.T0 ? env.tb.T0 : (t > env.tb.T1 ? env.tb.T1 : t);
-    const double T = t;
+    double T, Tdef, rmd;
+    const double2 *row;
+    bool ok;
+    L::prologue(u, env, T, row, Tdef, ok, rmd);
     c.T = T;
-    const double logT = plog(t);
-    const int bin_id = (int)(env.tb.idbin * (logT - env.tb.lb));
-    const int b = bin_id <= 0 ? 0 : (bin_id >= env.tb.nbins ? env.tb.nbins - 1 : bin_id);
-    const double Tdef = (logT - (env.tb.lb + bin_id * env.tb.dbin)) * env.tb.idbin;
-    const double2 *row = env.tb.pairs + (size_t)b * NTAB;
-    auto rate = [&](int k) {
-        const double2 v = row[k];
-        return v.x + Tdef * (v.y - v.x);
-    };
-    bool ok = true;
-#pragma unroll
-    for (int s = 0; s < N; ++s) ok = ok && !(u[s] < 0.0);
-    const double md = (u[H2I] + u[H2II] + u[HI] + u[HII] + u[HM] + u[HeI] + u[HeII] + u[HeIII])
-                    * 1.67e-24;
-    const double rmd = 1.0 / md;
+    auto rate = [&](auto K) { return L::template rate<CV(K)>(row, Tdef, env); };
 
     // species equations and their Jacobian, one reaction at a time: A
     // accumulates the column-scaled Jacobian sum_r nu * d(flux_r)/dn_col * sc_col
@@ -162,76 +164,76 @@ DENGO_D int sweep(LeanCell &c, const KParams<PrimNet> &p)
     for (int i = 0; i < M; ++i)
 #pragma unroll
         for (int j = 0; j < M; ++j) A[i][j] = 0.0;
-    static_for<0, NRXN>([&](auto r) {
-        constexpr Rxn R = RXN[CV(r)];
-        const double k = rate(R.table);
+    static_for<0, L::NRXN>([&](auto r) {
+        constexpr int R = CV(r);
+        const double k = rate(std::integral_constant<int, L::table(R)>{});
         double flux = k;
-        static_for<0, R.nslot>([&](auto q) { flux = flux * u[R.slot[CV(q)]]; });
+        static_for<0, L::nslot(R)>([&](auto q) { flux = flux * u[L::slot(R, CV(q))]; });
         static_for<0, GE>([&](auto s) {
-            constexpr int v = nu(CV(r), CV(s));
+            constexpr int v = L::nu(R, CV(s));
             if constexpr (v != 0) f[CV(s)] += (double)v * flux;
         });
-        static_for<0, R.nslot>([&](auto j) {
+        static_for<0, L::nslot(R)>([&](auto j) {
             double part = k;
-            static_for<0, R.nslot>([&](auto q) {
+            static_for<0, L::nslot(R)>([&](auto q) {
                 if constexpr (CV(q) != CV(j))
-                    part = part * u[R.slot[CV(q)]];
+                    part = part * u[L::slot(R, CV(q))];
             });
-            constexpr int col = R.slot[CV(j)];
-            part = part * c.at(L_SC + col);
+            constexpr int col = L::slot(R, CV(j));
+            part = part * c.at(C::L_SC + col);
             static_for<0, GE>([&](auto s) {
-                constexpr int v = nu(CV(r), CV(s));
+                constexpr int v = L::nu(R, CV(s));
                 if constexpr (v != 0) A[CV(s)][col] += (double)v * part;
             });
         });
     });
     if (!ok) return SW_FAILED;
 
-    // Newton matrix and right-hand side (sweep_fast / prim::jac_fast)
+    // Newton matrix and right-hand side (sweep_fast / jac_fast)
     double y[M];
     {
         const double dt = c.dt;
         static_for<0, M>([&](auto ii) {
             constexpr int i = CV(ii);
-            const double inv = c.at(L_INV + i);
+            const double inv = c.at(C::L_INV + i);
             const double dtinv = dt * inv;
             y[i] = (u[i] - u0(c, i)) * inv - dtinv * f[i];
             static_for<0, M>([&](auto jj) {
                 constexpr int j = CV(jj);
                 if constexpr (i == j) A[i][j] = 1.0 - dtinv * A[i][j];
-                else if constexpr (kPlan.S[0][i][j]) A[i][j] = -dtinv * A[i][j];
+                else if constexpr (L::plan.S[0][i][j]) A[i][j] = -dtinv * A[i][j];
                 else A[i][j] = 0.0;
             });
         });
     }
 
-    // block LU (be_fast.h block_lu_factor with the plan kPlan) with the
-    // forward substitution of block_lu_apply carried along as column M;
-    // A[k][k] takes the reciprocal pivot
+    // block LU (be_fast.h block_lu_factor with the plan L::plan) with the forward
+    // substitution of block_lu_apply carried along as column M; A[k][k] takes
+    // the reciprocal pivot
     static_for<0, M>([&](auto kk) {
         constexpr int k = CV(kk);
         if constexpr (k < M - 1) {
-            int p = k;
+            int pv = k;
             double best = fabs(A[k][k]);
             static_for<k + 1, M>([&](auto ii) {
                 constexpr int i = CV(ii);
-                if constexpr (kPlan.cand[k][i]) {
+                if constexpr (L::plan.cand[k][i]) {
                     const double v = fabs(A[i][k]);
                     if (v > best) {
                         best = v;
-                        p = i;
+                        pv = i;
                     }
                 }
             });
-            if (__any(p != k)) {
+            if (__any(pv != k)) {
                 static_for<k + 1, M>([&](auto ii) {
                     constexpr int i = CV(ii);
-                    if constexpr (kPlan.cand[k][i]) {
-                        const bool sw = (p == i);
+                    if constexpr (L::plan.cand[k][i]) {
+                        const bool sw = (pv == i);
                         if (__any(sw)) {
                             static_for<k, M>([&](auto jj) {
                                 constexpr int j = CV(jj);
-                                if constexpr (kPlan.S[k][k][j] || kPlan.S[k][i][j]) {
+                                if constexpr (L::plan.S[k][k][j] || L::plan.S[k][i][j]) {
                                     const double a = A[k][j], e = A[i][j];
                                     A[k][j] = sw ? e : a;
                                     A[i][j] = sw ? a : e;
@@ -250,11 +252,11 @@ DENGO_D int sweep(LeanCell &c, const KParams<PrimNet> &p)
         if constexpr (k < M - 1) {
             static_for<k + 1, M>([&](auto ii) {
                 constexpr int i = CV(ii);
-                if constexpr (kPlan.cand[k][i]) {
+                if constexpr (L::plan.cand[k][i]) {
                     const double m = A[i][k] * rp;
                     static_for<k + 1, M>([&](auto jj) {
                         constexpr int j = CV(jj);
-                        if constexpr (kPlan.piv[k][j]) A[i][j] = A[i][j] - m * A[k][j];
+                        if constexpr (L::plan.piv[k][j]) A[i][j] = A[i][j] - m * A[k][j];
                     });
                     y[i] = y[i] - m * y[k];
                 }
@@ -266,41 +268,24 @@ DENGO_D int sweep(LeanCell &c, const KParams<PrimNet> &p)
         constexpr int i = M - 1 - CV(rr);
         static_for<i + 1, M>([&](auto jj) {
             constexpr int j = CV(jj);
-            if constexpr (kPlan.fin[i][j]) y[i] = y[i] - A[i][j] * y[j];
+            if constexpr (L::plan.fin[i][j]) y[i] = y[i] - A[i][j] * y[j];
         });
         y[i] = y[i] * A[i][i];
     });
-    // energy equation and the energy row of the Newton matrix from the
-    // cooling rates (tables BREM..REHII follow the 22 reaction tables): only
-    // the energy's Newton step needs them, so they are evaluated after the
-    // species solve, when the factor's registers are free
+
+    // the energy's Newton step from the energy row: the cooling sum over the
+    // mass density and its gradient by species (the BE matrix's energy row)
     double yge;
     {
-        double rs[NTAB];
-#pragma unroll
-        for (int k = BREM; k < NTAB; ++k) rs[k] = rate(k);
-        const double *cr = rs;
-        const double de = u[DE];
-        const double galdl = cr[GAHI] * u[HI] + cr[GAH2] * u[H2I] + cr[GAHE] * u[HeI]
-                           + cr[GAHP] * u[HII] + cr[GAEL] * de;
-        const double cool =
-            -de * (cr[BREM] * (u[HII] + u[HeII] + 4.0 * u[HeIII])
-                   + u[HeII] * ((cr[CEHEI] + cr[CIHEIS]) * de + cr[CEHEII] + cr[CIHEII]
-                                + cr[REHEII1] + cr[REHEII2])
-                   + u[HI] * (cr[CEHI] + cr[CIHI]) + cr[CIHEI] * u[HeI]
-                   + (cr[COMP] * env.comp4) * (T - 2.73 * env.zp1)
-                   + cr[REHEIII] * u[HeIII] + cr[REHII] * u[HII])
-            - u[H2I] * cr[GPHDL] / (1.0 + cr[GPHDL] / galdl)
-            + u[HI] * (cr[H2MHEAT] * u[HI] * u[HI] - cr[H2MCOOL] * u[H2I]);
-        const double fge = cool * rmd;
         double g[N];
-        cooling_grad(rs, u, T, env.zp1, env.comp4, g);
-        const double inv = c.at(L_INV + GE);
+        const double cool = L::energy(u, rate, T, env, g);
+        const double fge = cool * rmd;
+        const double inv = c.at(C::L_INV + GE);
         const double dtinv = c.dt * inv;
         const double w = -dtinv * rmd;
         yge = (u[GE] - u0(c, GE)) * inv - dtinv * fge;
 #pragma unroll
-        for (int j = 0; j < M; ++j) yge = yge - (w * g[j] * c.at(L_SC + j)) * y[j];
+        for (int j = 0; j < M; ++j) yge = yge - (w * g[j] * c.at(C::L_SC + j)) * y[j];
     }
 
     // u -= scale * y; max-norm test in scaled units (first unconverged entry)
@@ -309,15 +294,15 @@ DENGO_D int sweep(LeanCell &c, const KParams<PrimNet> &p)
     const double *rtrow = p.rtol + ((c.flags & kOwnRtol) ? c.id * N : 0);
     double rtol[N];
 #pragma unroll
-    for (int i = 0; i < N; ++i) rtol[i] = rtrow[i];
+    for (int i = 0; i < N; ++i) rtol[i] = rtrow[L::ext(i)];
     bool unsolved = false, nan = false;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
         const double yi = i < M ? y[i] : yge;
-        const double inv = c.at(L_INV + i);
-        c.u[i] = c.u[i] - c.at(L_SC + i) * yi;
+        const double inv = c.at(C::L_INV + i);
+        c.u[i] = c.u[i] - c.at(C::L_SC + i) * yi;
         if (!unsolved && !nan) {
-            if (fabs(yi) > (c.at(L_ATOL + i) + rtol[i] * fabs(c.u[i] * inv))) unsolved = true;
+            if (fabs(yi) > (c.at(C::L_ATOL + i) + rtol[i] * fabs(c.u[i] * inv))) unsolved = true;
             else if (c.u[i] != c.u[i]) nan = true;
         }
     }
@@ -328,11 +313,14 @@ DENGO_D int sweep(LeanCell &c, const KParams<PrimNet> &p)
 }
 
 // BE exit and one pass of the substep loop (be_fast.h end_be_fast).
-DENGO_D bool end_be(LeanCell &c, int code, const KParams<PrimNet> &p)
+template <class L>
+DENGO_D bool end_be(LeanCell<L> &c, int code, const KParams<typename L::Net> &p)
 {
+    constexpr int N = L::N;
+    using C = LeanCell<L>;
     int rv = code == SW_CONVERGED ? 0 : 1;
 #pragma unroll
-    for (int i = 0; i < N; ++i) c.at(L_ATOL + i) *= c.at(L_SC + i);
+    for (int i = 0; i < N; ++i) c.at(C::L_ATOL + i) *= c.at(C::L_SC + i);
 #pragma unroll
     for (int i = 0; i < N; ++i)
         if (c.u[i] < 0 || c.u[i] != c.u[i]) rv = 1;
@@ -345,7 +333,7 @@ DENGO_D bool end_be(LeanCell &c, int code, const KParams<PrimNet> &p)
         c.ttot += c.dt;
         c.dt = (c.dt * 1.1 < p.dtf - c.ttot) ? c.dt * 1.1 : p.dtf - c.ttot;
 #pragma unroll
-        for (int i = 0; i < N; ++i) c.at(L_PREV + i) = c.u[i];
+        for (int i = 0; i < N; ++i) c.at(C::L_PREV + i) = c.u[i];
 #pragma unroll
         for (int i = 0; i < N; ++i)
             if (c.u[i] < 1e-25) c.u[i] = 1e-25;
@@ -354,7 +342,7 @@ DENGO_D bool end_be(LeanCell &c, int code, const KParams<PrimNet> &p)
         c.nrej++;
         c.dt /= 2.0;
 #pragma unroll
-        for (int i = 0; i < N; ++i) c.u[i] = c.at(L_PREV + i);
+        for (int i = 0; i < N; ++i) c.u[i] = c.at(C::L_PREV + i);
         c.flags &= ~kFloored;
         if (c.dt / p.dtf < 1e-15) {
             c.status = 1;
@@ -370,25 +358,28 @@ DENGO_D bool end_be(LeanCell &c, int code, const KParams<PrimNet> &p)
 }
 
 // dengo_evolve exit (be_driver.h finish_cell): number -> mass, stores.
-DENGO_D void finish_cell(LeanCell &c, const KParams<PrimNet> &p)
+template <class L>
+DENGO_D void finish_cell(LeanCell<L> &c, const KParams<typename L::Net> &p)
 {
+    constexpr int N = L::N, GE = L::GE;
+    using C = LeanCell<L>;
     double at[N];
 #pragma unroll
-    for (int k = 0; k < N; ++k) at[k] = c.at(L_ATOL + k);
+    for (int k = 0; k < N; ++k) at[k] = c.at(C::L_ATOL + k);
     if (!p.input_is_number) {
 #pragma unroll
         for (int k = 0; k < N; ++k) {
             if (k == GE) continue;
-            c.u[k] *= PrimNet::weight(k) * 1.67e-24;
-            at[k] *= PrimNet::weight(k) * 1.67e-24;
+            c.u[k] *= L::weight(k) * 1.67e-24;
+            at[k] *= L::weight(k) * 1.67e-24;
         }
     }
     double *out = p.input + c.id * N;
     double *atw = p.atol + c.id * N;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        out[k] = c.u[k];
-        atw[k] = at[k];
+        out[L::ext(k)] = c.u[k];
+        atw[L::ext(k)] = at[k];
     }
     if (p.status) p.status[c.id] = c.status;
     if (p.nsub) p.nsub[c.id] = c.siter;
@@ -399,7 +390,8 @@ DENGO_D void finish_cell(LeanCell &c, const KParams<PrimNet> &p)
     if (p.T_out) p.T_out[c.id] = c.T;
 }
 
-DENGO_D bool start_or_finish(LeanCell &c, long long id, const KParams<PrimNet> &p)
+template <class L>
+DENGO_D bool start_or_finish(LeanCell<L> &c, long long id, const KParams<typename L::Net> &p)
 {
     begin_cell(c, id, p);
     if (!(c.ttot < p.dtf)) {
@@ -410,17 +402,97 @@ DENGO_D bool start_or_finish(LeanCell &c, long long id, const KParams<PrimNet> &
     return true;
 }
 
+// ---------------------------------------------------------------------------
+// The primordial network (dengo_device.h): the same expressions as
+// prim::rhs_fast, prim::jac_fast and prim::cooling_grad.
+// ---------------------------------------------------------------------------
+struct LeanPrim {
+    using Net = PrimNet;
+    static constexpr int N = prim::N, M = prim::M, GE = prim::GE, DE = prim::DE;
+    static constexpr int NTAB = prim::NTAB, NRXN = prim::NRXN;
+    static constexpr const LUPlan<M> &plan = prim::kPlan;
+    static DENGO_D double weight(int k) { return PrimNet::weight(k); }
+    static DENGO_HD double free_e(int k) { return PrimNet::free_e(k); }
+    static constexpr int ext(int k) { return k; }
+    static constexpr int table(int r) { return prim::RXN[r].table; }
+    static constexpr int nslot(int r) { return prim::RXN[r].nslot; }
+    static constexpr int slot(int r, int q) { return prim::RXN[r].slot[q]; }
+    static constexpr int nu(int r, int s) { return prim::nu(r, s); }
+
+    // temperature, table row and offset, no negative species, 1/mass density
+    static DENGO_D void prologue(const double (&u)[N], const prim::Env &env, double &T,
+                                 const double2 *&row, double &Tdef, bool &ok, double &rmd)
+    {
+#pragma clang fp contract(fast)
+        using namespace prim;
+        const double density = 2.0 * (u[H2I] + u[H2II]) + 1.00794 * (u[HI] + u[HII] + u[HM])
+                             + 4.002602 * (u[HeI] + u[HeII] + u[HeIII]);
+        const double gsum = 2.5 * (u[H2I] + u[H2II])
+                          + 1.5 * (u[HI] + u[HII] + u[HM] + u[HeI] + u[HeII] + u[HeIII] + u[DE]);
+        double t = density * u[GE] * (1.67e-24 / 1.3806504e-16) / gsum;
+        t = t < env.tb.T0 ? env.tb.T0 : (t > env.tb.T1 ? env.tb.T1 : t);
+        T = t;
+        const double logT = plog(t);
+        const int bin_id = (int)(env.tb.idbin * (logT - env.tb.lb));
+        const int b = bin_id <= 0 ? 0 : (bin_id >= env.tb.nbins ? env.tb.nbins - 1 : bin_id);
+        Tdef = (logT - (env.tb.lb + bin_id * env.tb.dbin)) * env.tb.idbin;
+        row = env.tb.pairs + (size_t)b * NTAB;
+        ok = true;
+#pragma unroll
+        for (int s = 0; s < N; ++s) ok = ok && !(u[s] < 0.0);
+        const double md = (u[H2I] + u[H2II] + u[HI] + u[HII] + u[HM] + u[HeI] + u[HeII]
+                           + u[HeIII]) * 1.67e-24;
+        rmd = 1.0 / md;
+    }
+
+    template <int K>
+    static DENGO_D double rate(const double2 *row, double Tdef, const prim::Env &)
+    {
+#pragma clang fp contract(fast)
+        const double2 v = row[K];
+        return v.x + Tdef * (v.y - v.x);
+    }
+
+    // the cooling sum (prim::rhs_fast's energy row before 1/mdensity) and its
+    // gradient by species (prim::cooling_grad), from the cooling tables
+    template <class Rate>
+    static DENGO_D double energy(const double (&u)[N], Rate rate, double T, const prim::Env &env,
+                                 double (&g)[N])
+    {
+#pragma clang fp contract(fast)
+        using namespace prim;
+        double rs[NTAB];
+        static_for<BREM, NTAB>([&](auto k) { rs[CV(k)] = rate(k); });
+        const double *cr = rs;
+        const double de = u[DE];
+        const double galdl = cr[GAHI] * u[HI] + cr[GAH2] * u[H2I] + cr[GAHE] * u[HeI]
+                           + cr[GAHP] * u[HII] + cr[GAEL] * de;
+        const double cool =
+            -de * (cr[BREM] * (u[HII] + u[HeII] + 4.0 * u[HeIII])
+                   + u[HeII] * ((cr[CEHEI] + cr[CIHEIS]) * de + cr[CEHEII] + cr[CIHEII]
+                                + cr[REHEII1] + cr[REHEII2])
+                   + u[HI] * (cr[CEHI] + cr[CIHI]) + cr[CIHEI] * u[HeI]
+                   + (cr[COMP] * env.comp4) * (T - 2.73 * env.zp1)
+                   + cr[REHEIII] * u[HeIII] + cr[REHII] * u[HII])
+            - u[H2I] * cr[GPHDL] / (1.0 + cr[GPHDL] / galdl)
+            + u[HI] * (cr[H2MHEAT] * u[HI] * u[HI] - cr[H2MCOOL] * u[H2I]);
+        cooling_grad(rs, u, T, env.zp1, env.comp4, g);
+        return cool;
+    }
+};
+
 }  // namespace lean
 
-// The FP64-fast primordial kernel: one cell per lane, 64-lane blocks, two
-// waves per SIMD.  PERSIST as in k_chem: resident lanes pull cells from
-// p.counter (one atomic per wave), or one cell per lane over the whole grid.
-template <bool PERSIST>
+// The FP64-fast lean kernel: one cell per lane, 64-lane blocks, two waves per
+// SIMD.  PERSIST as in k_chem: resident lanes pull cells from p.counter (one
+// atomic per wave), or one cell per lane over the whole grid.
+template <class L, bool PERSIST>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
-k_chem_lean(KParams<PrimNet> p)
+k_chem_lean(KParams<typename L::Net> p)
 {
-    __shared__ double cold[lean::kSlots * lean::kLanes];
-    lean::LeanCell c;
+    using Cell = lean::LeanCell<L>;
+    __shared__ double cold[Cell::kSlots * lean::kLanes];
+    Cell c;
     const int lane = threadIdx.x & 63;
     c.cold = cold + lane;
     bool active = false;

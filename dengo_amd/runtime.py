@@ -368,9 +368,13 @@ class NetworkSolver(object):
             _p(res["nsweep"], _ip), _p(res["dt_cell"]), _p(res["T"]))
         if ttot < 0:
             raise RuntimeError("evolve failed: %s" % last_error(self.lib))
-        res.update(state=out, atol=at, ttot=ttot, dt=dtc.value,
-                   kernel_ms=load().dengo_last_kernel_ms())
+        res.update(state=out, atol=at, ttot=ttot, dt=dtc.value, kernel_ms=self.last_kernel_ms())
         return res
+
+    def last_kernel_ms(self):
+        """Kernel time of this solver's last synchronising evolve call (ms),
+        from the library that ran it (a generated solver library keeps its own)."""
+        return self.lib.dengo_last_kernel_ms()
 
     def evolve_device(self, inp, rtol, atol, dtf, dt=-1.0, z=-1.0, outputs=None,
                       stream=None, sync_min=True, **opt):
@@ -544,9 +548,13 @@ class SyntheticSolver(object):
             _p(res["nsweep"], _ip), _p(res["dt_cell"]))
         if ttot < 0:
             raise RuntimeError("evolve failed: %s" % last_error())
-        res.update(state=out, atol=at, ttot=ttot, dt=dtc.value,
-                   kernel_ms=load().dengo_last_kernel_ms())
+        res.update(state=out, atol=at, ttot=ttot, dt=dtc.value, kernel_ms=self.last_kernel_ms())
         return res
+
+    def last_kernel_ms(self):
+        """Kernel time of this solver's last synchronising evolve call (ms),
+        from the library that ran it (a generated solver library keeps its own)."""
+        return self.lib.dengo_last_kernel_ms()
 
     def evolve_device(self, x, T, rtol, atol, dtf, dt=-1.0, outputs=None, stream=None,
                       sync_min=True, **opt):

@@ -475,6 +475,39 @@ void gen_jac(const double *u, const double *rs, double T, double z, double *J)
            *reinterpret_cast<const double (*)[G::NTAB]>(rs), T, z,
            *reinterpret_cast<double (*)[G::N * G::N]>(J));
 }
+}
+// The forms of the register-lean sweep (LEAN): the species rows from the
+// reaction table as mass action, the cooling sum and its gradient.  Returns 0
+// for a network without them.
+template <class H>
+int lean_forms(const double *u, const double *rs, double T, double z, double *fs, double *cool,
+               double *grad, int *etab)
+{
+    if constexpr (H::LEAN) {
+        for (int s = 0; s < H::N; ++s) fs[s] = 0.0;
+        for (int r = 0; r < H::NRXN; ++r) {
+            double flux = rs[H::RXN_TAB[r]];
+            for (int q = 0; q < H::RXN_NSLOT[r]; ++q) flux = flux * u[H::RXN_SLOT[r][q]];
+            for (int s = 0; s < H::N; ++s) fs[s] = fs[s] + H::NU[r][s] * flux;
+        }
+        const auto &ua = *reinterpret_cast<const double (*)[H::N]>(u);
+        const auto &ra = *reinterpret_cast<const double (*)[H::NTAB]>(rs);
+        *cool = H::cool(ua, ra, T, z);
+        H::cool_grad(ua, ra, T, z, *reinterpret_cast<double (*)[H::N]>(grad));
+        for (int t = 0; t < H::NTAB; ++t) etab[t] = H::ETAB[t] ? 1 : 0;
+        return 1;
+    } else {
+        return 0;
+    }
+}
+extern "C" {
+int gen_lean(const double *u, const double *rs, double T, double z, double *fs, double *cool,
+             double *grad, int *etab)
+{
+    return lean_forms<G>(u, rs, T, z, fs, cool, grad, etab);
+}
+}
+extern "C" {
 void gen_layout(int *n, int *ntab, int *de, double *W, double *fe, int *h2, int *zslot,
                 double *zb, int *nz, int *ext)
 {
@@ -571,6 +604,29 @@ def _to_external(a, ext):
     out = np.empty_like(a)
     out[:, ext] = a
     return out
+
+
+def gen_lean(net, u, rs, T, z):
+    """The generated network's forms of the register-lean sweep, per row of
+    internal-order states u with table values rs and temperatures T: (species
+    rows from the reaction table [n, N], cooling sum [n], its gradient
+    [n, N], ETAB mask), or None for a network without them."""
+    f = net._lib.gen_lean
+    f.restype = ctypes.c_int
+    f.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _ip]
+    n, N = u.shape
+    fs, cool, grad = np.zeros((n, N)), np.zeros(n), np.zeros((n, N))
+    etab = np.zeros(net.ntab, dtype=np.int32)
+    for i in range(n):
+        ui = np.ascontiguousarray(u[i], dtype=np.float64)
+        ri = np.ascontiguousarray(rs[i], dtype=np.float64)
+        c = np.zeros(1)
+        fi, gi = np.zeros(N), np.zeros(N)
+        if not f(_ptr(ui), _ptr(ri), float(T[i]), float(z), _ptr(fi), _ptr(c), _ptr(gi),
+                 _ptr(etab, _ip)):
+            return None
+        fs[i], cool[i], grad[i] = fi, c[0], gi
+    return fs, cool, grad, etab.astype(bool)
 
 
 def gen_eval(net, u, tables, T_bounds, z=-1.0, flags=0):

@@ -258,6 +258,44 @@ struct Gen {
         J[119] = 0.0;
         J[120] = 0.0;
     }
+
+    // register-lean FP64-fast sweep (lean_sweep.h): reaction table of the species
+    // rows, checked against them symbolically, and the energy row alone
+    static constexpr bool LEAN = true;
+    static constexpr int NRXN = 24, MAXSLOT = 2;
+    static constexpr int RXN_TAB[NRXN] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23};
+    static constexpr int RXN_NSLOT[NRXN] = {2, 1, 2, 1, 2, 2, 1, 2, 2, 1, 2, 2, 1, 2, 2, 1, 2, 2, 1, 2, 2, 1, 2, 2};
+    static constexpr int RXN_SLOT[NRXN][MAXSLOT] = {{0, 9}, {0, 0}, {1, 9}, {1, 0}, {1, 9}, {2, 9}, {2, 0}, {2, 9}, {3, 9}, {3, 0}, {3, 9}, {4, 9}, {4, 0}, {4, 9}, {5, 9}, {5, 0}, {5, 9}, {6, 9}, {6, 0}, {6, 9}, {7, 9}, {7, 0}, {7, 9}, {8, 9}};
+    static constexpr int NU[NRXN][N] = {{-1, 1, 0, 0, 0, 0, 0, 0, 0, 1, 0}, {-1, 1, 0, 0, 0, 0, 0, 0, 0, 1, 0}, {0, -1, 1, 0, 0, 0, 0, 0, 0, 1, 0}, {0, -1, 1, 0, 0, 0, 0, 0, 0, 1, 0}, {1, -1, 0, 0, 0, 0, 0, 0, 0, -1, 0}, {0, 0, -1, 1, 0, 0, 0, 0, 0, 1, 0}, {0, 0, -1, 1, 0, 0, 0, 0, 0, 1, 0}, {0, 1, -1, 0, 0, 0, 0, 0, 0, -1, 0}, {0, 0, 0, -1, 1, 0, 0, 0, 0, 1, 0}, {0, 0, 0, -1, 1, 0, 0, 0, 0, 1, 0}, {0, 0, 1, -1, 0, 0, 0, 0, 0, -1, 0}, {0, 0, 0, 0, -1, 1, 0, 0, 0, 1, 0}, {0, 0, 0, 0, -1, 1, 0, 0, 0, 1, 0}, {0, 0, 0, 1, -1, 0, 0, 0, 0, -1, 0}, {0, 0, 0, 0, 0, -1, 1, 0, 0, 1, 0}, {0, 0, 0, 0, 0, -1, 1, 0, 0, 1, 0}, {0, 0, 0, 0, 1, -1, 0, 0, 0, -1, 0}, {0, 0, 0, 0, 0, 0, -1, 1, 0, 1, 0}, {0, 0, 0, 0, 0, 0, -1, 1, 0, 1, 0}, {0, 0, 0, 0, 0, 1, -1, 0, 0, -1, 0}, {0, 0, 0, 0, 0, 0, 0, -1, 1, 1, 0}, {0, 0, 0, 0, 0, 0, 0, -1, 1, 1, 0}, {0, 0, 0, 0, 0, 0, 1, -1, 0, -1, 0}, {0, 0, 0, 0, 0, 0, 0, 1, -1, -1, 0}};
+    // tables the energy row reads
+    static constexpr bool ETAB[NTAB] = {false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true};
+
+    static DENGO_GEN_FN double cool(const double (&u)[N], const double (&rs)[NTAB], double T, double z)
+    {
+        (void)T;
+        (void)z;
+        double e;
+        e = -u[0]*rs[24]*u[9] + u[0]*rs[25] - u[1]*rs[26]*u[9] + u[1]*rs[27] - u[2]*rs[28]*u[9] + u[2]*rs[29] - u[3]*rs[30]*u[9] + u[3]*rs[31] - u[4]*rs[32]*u[9] + u[4]*rs[33] - u[5]*rs[34]*u[9] + u[5]*rs[35] - u[6]*rs[36]*u[9] + u[6]*rs[37] - u[7]*rs[38]*u[9] + u[7]*rs[39] - u[8]*rs[40]*u[9];
+        return e;
+    }
+
+    static DENGO_GEN_FN void cool_grad(const double (&u)[N], const double (&rs)[NTAB], double T,
+                                       double z, double (&g)[N])
+    {
+        (void)T;
+        (void)z;
+        g[10] = 0.0;
+        g[0] = -rs[24]*u[9] + rs[25];
+        g[1] = -rs[26]*u[9] + rs[27];
+        g[2] = -rs[28]*u[9] + rs[29];
+        g[3] = -rs[30]*u[9] + rs[31];
+        g[4] = -rs[32]*u[9] + rs[33];
+        g[5] = -rs[34]*u[9] + rs[35];
+        g[6] = -rs[36]*u[9] + rs[37];
+        g[7] = -rs[38]*u[9] + rs[39];
+        g[8] = -rs[40]*u[9];
+        g[9] = -u[0]*rs[24] - u[1]*rs[26] - u[2]*rs[28] - u[3]*rs[30] - u[4]*rs[32] - u[5]*rs[34] - u[6]*rs[36] - u[7]*rs[38] - u[8]*rs[40];
+    }
 };
 }  // namespace net_oxydata
 }  // namespace dengo

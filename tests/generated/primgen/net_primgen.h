@@ -248,6 +248,54 @@ struct Gen {
         J[98] = 0.0;
         J[99] = 0.0;
     }
+
+    // register-lean FP64-fast sweep (lean_sweep.h): reaction table of the species
+    // rows, checked against them symbolically, and the energy row alone
+    static constexpr bool LEAN = true;
+    static constexpr int NRXN = 22, MAXSLOT = 3;
+    static constexpr int RXN_TAB[NRXN] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21};
+    static constexpr int RXN_NSLOT[NRXN] = {2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 3, 3, 2};
+    static constexpr int RXN_SLOT[NRXN][MAXSLOT] = {{2, 8, 0}, {3, 8, 0}, {5, 8, 0}, {6, 8, 0}, {6, 8, 0}, {7, 8, 0}, {2, 8, 0}, {4, 2, 0}, {2, 3, 0}, {1, 2, 0}, {0, 3, 0}, {0, 8, 0}, {0, 2, 0}, {4, 8, 0}, {4, 2, 0}, {4, 3, 0}, {4, 3, 0}, {1, 8, 0}, {1, 4, 0}, {2, 2, 0}, {2, 2, 2}, {0, 0, 0}};
+    static constexpr int NU[NRXN][N] = {{0, 0, -1, 1, 0, 0, 0, 0, 1, 0}, {0, 0, 1, -1, 0, 0, 0, 0, -1, 0}, {0, 0, 0, 0, 0, -1, 1, 0, 1, 0}, {0, 0, 0, 0, 0, 1, -1, 0, -1, 0}, {0, 0, 0, 0, 0, 0, -1, 1, 1, 0}, {0, 0, 0, 0, 0, 0, 1, -1, -1, 0}, {0, 0, -1, 0, 1, 0, 0, 0, -1, 0}, {1, 0, -1, 0, -1, 0, 0, 0, 1, 0}, {0, 1, -1, -1, 0, 0, 0, 0, 0, 0}, {1, -1, -1, 1, 0, 0, 0, 0, 0, 0}, {-1, 1, 1, -1, 0, 0, 0, 0, 0, 0}, {-1, 0, 0, 2, 0, 0, 0, 0, 0, 0}, {-1, 0, 2, 0, 0, 0, 0, 0, 0, 0}, {0, 0, 1, 0, -1, 0, 0, 0, 1, 0}, {0, 0, 1, 0, -1, 0, 0, 0, 1, 0}, {0, 0, 2, -1, -1, 0, 0, 0, 0, 0}, {0, 1, 0, -1, -1, 0, 0, 0, 1, 0}, {0, -1, 2, 0, 0, 0, 0, 0, -1, 0}, {1, -1, 1, 0, -1, 0, 0, 0, 0, 0}, {1, 0, -2, 0, 0, 0, 0, 0, 0, 0}, {1, 0, -2, 0, 0, 0, 0, 0, 0, 0}, {-1, 0, 2, 0, 0, 0, 0, 0, 0, 0}};
+    // tables the energy row reads
+    static constexpr bool ETAB[NTAB] = {false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, false, true, true, true, true, true, true, true, true, true, false, true, true, true, true, true, true, false, true, true, true, true, true, true};
+
+    static DENGO_GEN_FN double cool(const double (&u)[N], const double (&rs)[NTAB], double T, double z)
+    {
+        (void)T;
+        (void)z;
+        double e;
+        const double c0 = u[2]*u[8];
+        const double c1 = u[6]*u[8];
+        const double c2 = u[6]*(u[8]*u[8]);
+        e = -u[0]*u[2]*rs[39] - u[0]*rs[37]/(rs[37]/(u[0]*rs[33] + u[2]*rs[35] + u[3]*rs[36] + u[5]*rs[34] + u[8]*rs[32]) + 1.0) + (u[2]*u[2]*u[2])*rs[40] - u[3]*u[8]*rs[44] - u[5]*rs[26]*u[8] - u[7]*u[8]*rs[43] - rs[22]*u[8]*(u[3] + u[6] + 4.0*u[7]) - c0*rs[25] - c0*rs[29] - c1*rs[24] - c1*rs[27] - c1*rs[41] - c1*rs[42] - c2*rs[23] - c2*rs[28] + rs[30]*u[8]*((z + 1.0)*(z + 1.0)*(z + 1.0)*(z + 1.0))*(-T + 2.73*z + 2.73);
+        return e;
+    }
+
+    static DENGO_GEN_FN void cool_grad(const double (&u)[N], const double (&rs)[NTAB], double T,
+                                       double z, double (&g)[N])
+    {
+        (void)T;
+        (void)z;
+        g[9] = 0.0;
+        const double c0 = u[0]*rs[33];
+        const double c1 = u[2]*rs[35] + u[3]*rs[36] + u[5]*rs[34] + c0 + u[8]*rs[32];
+        const double c2 = 1.0 + rs[37]/c1;
+        const double c3 = (rs[37]*rs[37])/((c1*c1)*(c2*c2));
+        const double c4 = u[0]*c3;
+        const double c5 = rs[23]*u[8];
+        const double c6 = rs[28]*u[8];
+        const double c7 = 2*u[6];
+        g[0] = -u[2]*rs[39] - c0*c3 - rs[37]/c2;
+        g[1] = 0;
+        g[2] = -u[0]*rs[39] + 3*(u[2]*u[2])*rs[40] - c4*rs[35] - rs[25]*u[8] - rs[29]*u[8];
+        g[3] = -rs[22]*u[8] - c4*rs[36] - u[8]*rs[44];
+        g[4] = 0;
+        g[5] = -c4*rs[34] - rs[26]*u[8];
+        g[6] = -u[8]*(rs[22] + c5 + c6 + rs[24] + rs[27] + rs[41] + rs[42]);
+        g[7] = -u[8]*(4.0*rs[22] + rs[43]);
+        g[8] = -u[2]*rs[25] - u[2]*rs[29] - u[3]*rs[44] - u[5]*rs[26] - u[6]*rs[24] - u[6]*rs[27] - u[6]*rs[41] - u[6]*rs[42] - u[7]*rs[43] - rs[22]*(u[3] + u[6] + 4.0*u[7]) - c4*rs[32] - c5*c7 - c6*c7 + rs[30]*((z + 1.0)*(z + 1.0)*(z + 1.0)*(z + 1.0))*(-T + 2.73*z + 2.73);
+    }
 };
 }  // namespace net_primgen
 }  // namespace dengo

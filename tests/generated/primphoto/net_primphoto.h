@@ -206,6 +206,50 @@ struct Gen {
         J[98] = 0.0;
         J[99] = 0.0;
     }
+
+    // register-lean FP64-fast sweep (lean_sweep.h): reaction table of the species
+    // rows, checked against them symbolically, and the energy row alone
+    static constexpr bool LEAN = true;
+    static constexpr int NRXN = 11, MAXSLOT = 2;
+    static constexpr int RXN_TAB[NRXN] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10};
+    static constexpr int RXN_NSLOT[NRXN] = {1, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2};
+    static constexpr int RXN_SLOT[NRXN][MAXSLOT] = {{2, 0}, {2, 8}, {3, 8}, {5, 8}, {6, 8}, {6, 8}, {7, 8}, {2, 8}, {4, 2}, {2, 3}, {1, 2}};
+    static constexpr int NU[NRXN][N] = {{0, 0, -1, 1, 0, 0, 0, 0, 1, 0}, {0, 0, -1, 1, 0, 0, 0, 0, 1, 0}, {0, 0, 1, -1, 0, 0, 0, 0, -1, 0}, {0, 0, 0, 0, 0, -1, 1, 0, 1, 0}, {0, 0, 0, 0, 0, 1, -1, 0, -1, 0}, {0, 0, 0, 0, 0, 0, -1, 1, 1, 0}, {0, 0, 0, 0, 0, 0, 1, -1, -1, 0}, {0, 0, -1, 0, 1, 0, 0, 0, -1, 0}, {1, 0, -1, 0, -1, 0, 0, 0, 1, 0}, {0, 1, -1, -1, 0, 0, 0, 0, 0, 0}, {1, -1, -1, 1, 0, 0, 0, 0, 0, 0}};
+    // tables the energy row reads
+    static constexpr bool ETAB[NTAB] = {false, false, false, false, false, false, false, false, false, false, false, true, true, true, true, true, true, true, true, true, false, true, true};
+
+    static DENGO_GEN_FN double cool(const double (&u)[N], const double (&rs)[NTAB], double T, double z)
+    {
+        (void)T;
+        (void)z;
+        double e;
+        e = -u[0]*rs[19]/(rs[19]/(u[0]*rs[15] + u[2]*rs[17] + u[3]*rs[18] + u[5]*rs[16] + u[8]*rs[14]) + 1.0) + u[2]*rs[21] - u[2]*rs[12]*u[8] - u[3]*u[8]*rs[22] - rs[11]*u[8]*(u[3] + u[6] + 4.0*u[7]) + rs[13]*u[8]*((z + 1.0)*(z + 1.0)*(z + 1.0)*(z + 1.0))*(-T + 2.73*z + 2.73);
+        return e;
+    }
+
+    static DENGO_GEN_FN void cool_grad(const double (&u)[N], const double (&rs)[NTAB], double T,
+                                       double z, double (&g)[N])
+    {
+        (void)T;
+        (void)z;
+        g[9] = 0.0;
+        const double c0 = u[0]*rs[15];
+        const double c1 = u[2]*rs[17] + u[3]*rs[18] + u[5]*rs[16] + c0 + u[8]*rs[14];
+        const double c2 = (1.0/(c1*c1));
+        const double c3 = 1.0 + rs[19]/c1;
+        const double c4 = rs[19]/c3;
+        const double c5 = u[0]*c2*(rs[19]*rs[19])/(c3*c3);
+        const double c6 = rs[11]*u[8];
+        g[0] = -c4*(c0*c2*c4 + 1);
+        g[1] = 0;
+        g[2] = rs[21] - c5*rs[17] - rs[12]*u[8];
+        g[3] = -c5*rs[18] - c6 - u[8]*rs[22];
+        g[4] = 0;
+        g[5] = -c5*rs[16];
+        g[6] = -c6;
+        g[7] = -4.0*c6;
+        g[8] = -u[2]*rs[12] - u[3]*rs[22] - rs[11]*(u[3] + u[6] + 4.0*u[7]) - c5*rs[14] + rs[13]*((z + 1.0)*(z + 1.0)*(z + 1.0)*(z + 1.0))*(-T + 2.73*z + 2.73);
+    }
 };
 }  // namespace net_primphoto
 }  // namespace dengo

@@ -188,3 +188,38 @@ def test_generated_primordial_evolves_like_the_hand_written_oracle(prim_full):
     rel = np.abs(a["state"] - b["state"]) / np.maximum(np.abs(a["state"]), 1e-300)
     assert np.mean(rel[done].max(axis=1) <= 1e-6) >= 0.9
     assert np.mean(a["nsweep"] == b["nsweep"]) >= 0.5
+
+
+@pytest.mark.parametrize("which", ["primgen", "primphoto"])
+def test_lean_forms_match_the_printed_rows(prim_full, photo, which):
+    """What the register-lean sweep evaluates instead of the printed rhs and
+    jac, compiled for the host: the species rows as mass action from the
+    printed reaction table, the energy row's cooling sum (cool) and its
+    gradient by species (cool_grad); ETAB lists every table they read."""
+    net, tab = prim_full if which == "primgen" else photo
+    g = oracle.gen_net(which)
+    _, n, _ = primordial_ics(200, seed=54, T_range=(15.0, 5e8))
+    z = 3.25
+    f, J, T, rs, _ = oracle.gen_eval(g, n, tab, TB, z=z)
+    ni = n[:, np.argsort(g.ext)] if not np.array_equal(g.ext, np.arange(len(g.ext))) else n
+    forms = oracle.gen_lean(g, ni, rs, T, z)
+    assert forms is not None, "the generated network has no lean forms"
+    fs, cool, grad, etab = forms
+    md = n[:, :8].sum(axis=1) * 1.67e-24
+    assert _row_close(fs[:, :GE], f[:, :GE], 1e-12)
+    assert np.allclose(cool / md, f[:, GE], rtol=1e-12, atol=0)
+    # J[c, col, row]: the energy row's entries by species column
+    assert _row_close(grad[:, :GE] / md[:, None], J[:, :GE, GE], 1e-12)
+    # the tables outside ETAB do not enter the energy row
+    masked = np.where(etab[None, :], rs, 0.0)
+    _, cool2, grad2, _ = oracle.gen_lean(g, ni, masked, T, z)
+    assert np.array_equal(cool2, cool) and np.array_equal(grad2, grad)
+    # and the reaction tables are the first ones, the cooling tables the rest
+    assert not etab[:len(net.ordered_reactions())].any()
+
+
+def test_lean_forms_in_the_printed_header(prim_full):
+    net, _ = prim_full
+    text = codegen.render_gen_network(net, "primgen_check")
+    assert "static constexpr bool LEAN = true;" in text
+    assert "static constexpr int NRXN = %d" % len(net.ordered_reactions()) in text
