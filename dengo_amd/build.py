@@ -15,7 +15,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libdengo_hip.so")
 SOURCES = ["dengo_common.hip", "dengo_primordial.hip", "dengo_oxygen.hip", "dengo_synthetic.hip",
            "dengo_server.hip", "dengo_graph.hip", "dengo_h5.hip", "dengo_noalloc.hip"]
-HEADERS = ["dengo_device.h", "be_driver.h", "be_fast.h", "ma_net.h", "dengo_solver.h", "net_oxygen.h",
+HEADERS = ["dengo_device.h", "be_driver.h", "be_fast.h", "lean_sweep.h", "gen_net.h", "ma_net.h", "dengo_solver.h", "net_oxygen.h",
            "big_net.h", "big_program.h", "arena.h"]
 
 HIPCC_FLAGS = [

@@ -290,15 +290,6 @@ struct LeanGen {
     }
 };
 
-// The lean adapter of a network (void: none)
-template <class Net, class = void>
-struct LeanOf {
-    using type = void;
-};
-template <>
-struct LeanOf<PrimNet> {
-    using type = LeanPrim;
-};
 template <class G>
 struct LeanOf<GenNet<G>, std::enable_if_t<G::LEAN>> {
     using type = LeanGen<G>;

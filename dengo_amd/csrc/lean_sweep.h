@@ -32,12 +32,13 @@
 // * a candidate row's exchange (a select per entry of both rows) runs only if
 //   some lane of the wave chose that row; the pivots are unchanged.
 //
-// A network enters through an adapter L (LeanPrim below; LeanGen<G> in
-// gen_net.h): its sizes, pivot plan, caller order and species weights; the
-// temperature, table row and 1/mass density of a state (prologue); a table's
-// interpolated value (rate<K>); its reactions (table, reactant slots, net
-// change per species); and its energy row (energy: the cooling sum, and its
-// gradient by species into g).
+// A network enters through an adapter L, found by lean::LeanOf<Net>
+// (LeanPrim below; LeanGen<G> in gen_net.h; LeanMA<D> in ma_net.h): its
+// sizes, pivot plan, caller order and species weights; the temperature, table
+// row and 1/mass density of a state (prologue); a table's interpolated value
+// (rate<K>); its reactions (table, reactant slots, net change per species);
+// and its energy row (energy: the cooling sum, and its gradient by species
+// into g).
 //
 // Reference semantics: BE_chem_solve (dengo/solvers/BE_chem_solve.C:79-241)
 // and the substep loop of dengo_evolve
@@ -481,6 +482,16 @@ struct LeanPrim {
     }
 };
 
+// The lean adapter of a network (void: none); gen_net.h and ma_net.h add
+// theirs.
+template <class Net, class = void>
+struct LeanOf {
+    using type = void;
+};
+template <>
+struct LeanOf<PrimNet> {
+    using type = LeanPrim;
+};
 }  // namespace lean
 
 // The FP64-fast lean kernel: one cell per lane, 64-lane blocks, two waves per

@@ -10,6 +10,7 @@
 // (interpolation), :460-522 (RHS), :527-599 (Jacobian).
 #pragma once
 #include "be_fast.h"
+#include "lean_sweep.h"
 
 namespace dengo {
 
@@ -311,5 +312,91 @@ struct FastTraits<MANet<D>> {
         Net::jac_fast(u, rs, T, rmd, sc, dtinv, env, A, age);
     }
 };
+
+namespace lean {
+// The register-lean FP64-fast sweep of a mass-action network (lean_sweep.h):
+// its reaction descriptor for the species rows, its cooling monomials for the
+// energy row, and the temperature, interpolation and mass density of
+// MANet<D>::rhs_fast.
+template <class D>
+struct LeanMA {
+    using Net = MANet<D>;
+    static constexpr int N = D::N, M = N - 1, GE = D::GE, DE = D::DE;
+    static constexpr int NTAB = D::NTAB, NRXN = D::NRXN;
+    static constexpr const LUPlan<M> &plan = Net::plan;
+    static constexpr int table(int r) { return D::RXN[r].table; }
+    static constexpr int nslot(int r) { return D::RXN[r].nslot; }
+    static constexpr int slot(int r, int q) { return D::RXN[r].slot[q]; }
+    static constexpr int nu(int r, int s) { return ma_nu<D>(r, s); }
+    static constexpr bool cooling_table(int t)
+    {
+        for (int c = 0; c < D::NCOOL; ++c)
+            if (D::COOL[c].table == t) return true;
+        return false;
+    }
+    static DENGO_D double weight(int k) { return D::W[k]; }
+    static DENGO_HD double free_e(int k) { return D::FREE_E[k]; }
+    static constexpr int ext(int k) { return D::EXT[k]; }
+
+    static DENGO_D void prologue(const double (&u)[N], const prim::Env &env, double &T,
+                                 const double2 *&row, double &Tdef, bool &ok, double &rmd)
+    {
+#pragma clang fp contract(fast)
+        double density = 0.0, gsum = 0.0, m = 0.0;
+        static_for<0, N>([&](auto s) {
+            if constexpr (CV(s) != GE && CV(s) != DE) {
+                density += D::W[CV(s)] * u[CV(s)];
+                m += u[CV(s)];
+            }
+            if constexpr (CV(s) != GE) gsum += u[CV(s)] * (D::H2G[CV(s)] ? 2.5 : 1.5);
+        });
+        double t = density * u[GE] * (1.67e-24 / 1.3806504e-16) / gsum;
+        t = t < env.tb.T0 ? env.tb.T0 : (t > env.tb.T1 ? env.tb.T1 : t);
+        T = t;
+        const double logT = plog(t);
+        const int bin_id = (int)(env.tb.idbin * (logT - env.tb.lb));
+        const int b = bin_id <= 0 ? 0 : (bin_id >= env.tb.nbins ? env.tb.nbins - 1 : bin_id);
+        Tdef = (logT - (env.tb.lb + bin_id * env.tb.dbin)) * env.tb.idbin;
+        row = env.tb.pairs + (size_t)b * NTAB;
+        ok = true;
+#pragma unroll
+        for (int s = 0; s < N; ++s) ok = ok && !(u[s] < 0.0);
+        rmd = 1.0 / (m * 1.67e-24);
+    }
+
+    template <int K>
+    static DENGO_D double rate(const double2 *row, double Tdef, const prim::Env &)
+    {
+#pragma clang fp contract(fast)
+        const double2 v = row[K];
+        return v.x + Tdef * (v.y - v.x);
+    }
+
+    template <class Rate>
+    static DENGO_D double energy(const double (&u)[N], Rate rate, double, const prim::Env &,
+                                 double (&g)[N])
+    {
+#pragma clang fp contract(fast)
+        double rs[NTAB];
+        static_for<0, NTAB>([&](auto k) {
+            if constexpr (cooling_table(CV(k))) rs[CV(k)] = rate(k);
+            else rs[CV(k)] = 0.0;
+        });
+        double sum = 0.0;
+        static_for<0, D::NCOOL>([&](auto c) {
+            constexpr CoolTerm C = D::COOL[CV(c)];
+            const double x = C.species >= 0 ? u[C.species >= 0 ? C.species : 0] : 1.0;
+            sum += C.coef * rs[C.table] * x * Net::depow(u[DE], C.depow);
+        });
+        Net::cool_grad(rs, u, g);
+        return sum;
+    }
+};
+
+template <class D>
+struct LeanOf<MANet<D>> {
+    using type = LeanMA<D>;
+};
+}  // namespace lean
 
 }  // namespace dengo

@@ -552,9 +552,8 @@ class SyntheticSolver(object):
         return res
 
     def last_kernel_ms(self):
-        """Kernel time of this solver's last synchronising evolve call (ms),
-        from the library that ran it (a generated solver library keeps its own)."""
-        return self.lib.dengo_last_kernel_ms()
+        """Kernel time of this solver's last synchronising evolve call (ms)."""
+        return load().dengo_last_kernel_ms()
 
     def evolve_device(self, x, T, rtol, atol, dtf, dt=-1.0, outputs=None, stream=None,
                       sync_min=True, **opt):
