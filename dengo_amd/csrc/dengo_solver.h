@@ -20,6 +20,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <atomic>
 #include <limits>
 #include <string>
 #include <type_traits>
@@ -930,8 +931,10 @@ struct Host {
     template <class LT>
     static bool lean_fits()
     {
-        static int fits = -1;   // per process: the kernels are the same on every device
-        if (fits < 0) {
+        // per process (the kernels are the same on every device); threads that
+        // race here compute the same answer
+        static std::atomic<int> fits{-1};
+        if (fits.load() < 0) {
             int ok = 1;
             for (const void *k : {(const void *)k_chem_lean<LT, true>,
                                   (const void *)k_chem_lean<LT, false>}) {
@@ -943,9 +946,9 @@ struct Host {
                     ok = 0;
             }
             (void)hipGetLastError();
-            fits = ok;
+            fits.store(ok);
         }
-        return fits == 1;
+        return fits.load() == 1;
     }
 
     template <int MODE>
