@@ -13,8 +13,12 @@ Variants timed (all on the same inputs):
                        per-cell sweep counts give the algorithmic work
   fp64_percell         the plain one-cell-per-lane FP64 kernel (drop-in baseline)
   fp64fast_persistent  headline by default (--variant): structure-exploiting
-                       FP64 block solve, persistent schedule
-  --also variants      timed once each (default: mixed_persistent)
+                       FP64 block solve, persistent schedule (the
+                       register-lean kernel, two waves per SIMD)
+  --also variants      timed once each (default: mixed_persistent and
+                       fp64fast_resident_persistent, the FP64-fast sweep as
+                       the register-resident kernel, bit-identical to the
+                       headline's)
 The CPU baseline is the oracle (plain C restatement, OpenMP over cells) on the
 first --cpu-cells cells of the same inputs, on the host's cores.
 
@@ -40,7 +44,7 @@ PEAK_FP64 = 78.6e12      # MI355X FP64 (vector and matrix) TFLOP/s
 # primordial workload: FETCH_SIZE + WRITE_SIZE from two rocprofv3 --pmc runs
 # of this script (--calib-copy-mib 1024), each scaled by the counter's
 # reading for the 1 GiB device copy: 1,037,038 + 621,080 KiB (means of the
-# two launches of each run; kernel k_chem_lean<true>).
+# two launches of each run; kernel k_chem_lean<lean::LeanPrim, true>).
 HEADLINE_TRAFFIC_BYTES = 1658118 * 1024
 PEAK_FP32 = 157.3e12     # MI355X FP32 vector
 
@@ -278,8 +282,9 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--variant", default="fp64fast_persistent",
                     choices=["fp64_persistent", "fp64_percell", "fp64fast_persistent",
-                             "fp64fast_percell", "mixed_persistent", "mixed_percell"])
-    ap.add_argument("--also", default="mixed_persistent",
+                             "fp64fast_percell", "mixed_persistent", "mixed_percell",
+                             "fp64fast_resident_persistent"])
+    ap.add_argument("--also", default="mixed_persistent,fp64fast_resident_persistent",
                     help="comma-separated extra variants timed once (empty: none)")
     ap.add_argument("--no-percell", action="store_true", help="skip the plain per-cell kernel")
     ap.add_argument("--no-cost-ordered", action="store_true",
@@ -342,8 +347,9 @@ def main():
         prec = (runtime.MIXED if variant.startswith("mixed") else
                 runtime.FP64_FAST if variant.startswith("fp64fast") else runtime.FP64)
         sched = runtime.PER_CELL if variant.endswith("percell") else runtime.PERSISTENT
+        form = 1 if "_resident" in variant else 0
         tmin, _ = solver.evolve_device(x, d_rtol, at, 1e12, outputs=outputs, stream=stream,
-                                       precision=prec, schedule=sched)
+                                       precision=prec, schedule=sched, kernel_form=form)
         return tmin, runtime.load().dengo_last_kernel_ms()
 
     if args.calib_copy_mib:
@@ -443,6 +449,7 @@ def main():
         extra[v] = {"kernel_ms": ms_e, "cell_solves_per_s": n / (ms_e * 1e-3)}
         rec, heads[v], heads[v + "_status"] = compare(xe, st_out.cpu().numpy())
         extra[v].update(rec)
+        extra[v]["bitwise_equal_to_" + args.variant] = bool(torch.equal(xe, xh))
         log("%s: %.1f ms kernel" % (v, ms_e))
     del xs, ats
 

@@ -473,6 +473,8 @@ inline int check_options(const dengo_options *opt, dengo_options *o)
     if (o->precision != DENGO_FP64 && o->precision != DENGO_MIXED &&
         o->precision != DENGO_FP64_FAST)
         return set_error("unknown precision %d", o->precision);
+    if (o->kernel_form != 0 && o->kernel_form != 1)
+        return set_error("kernel_form must be 0 or 1, got %d", o->kernel_form);
     return 0;
 }
 
@@ -924,7 +926,8 @@ struct Host {
     // whose lean kernel compiled without scratch and fits at least six 64-lane
     // workgroups per CU; otherwise, and for the other precisions, as the
     // register-resident k_chem.  The two forms of the primordial network are
-    // bit-identical.  Building with -DDENGO_LEAN=0 selects k_chem everywhere.
+    // bit-identical.  dengo_options.kernel_form = 1, or building with
+    // -DDENGO_LEAN=0, selects k_chem.
 #ifndef DENGO_LEAN
 #define DENGO_LEAN 1
 #endif
@@ -952,11 +955,11 @@ struct Host {
     }
 
     template <int MODE>
-    static void kernels(void (**kp)(KParams<Net>), void (**kc)(KParams<Net>))
+    static void kernels(void (**kp)(KParams<Net>), void (**kc)(KParams<Net>), int form)
     {
         using LT = typename lean::LeanOf<Net>::type;
         if constexpr (DENGO_LEAN && MODE == kFast64 && !std::is_void<LT>::value) {
-            if (lean_fits<LT>()) {
+            if (form == 0 && lean_fits<LT>()) {
                 *kp = k_chem_lean<LT, true>;
                 *kc = k_chem_lean<LT, false>;
                 return;
@@ -971,9 +974,9 @@ struct Host {
     {
         void (*kp)(KParams<Net>) = nullptr;
         void (*kc)(KParams<Net>) = nullptr;
-        if (o.precision == DENGO_FP64) kernels<kFaithful>(&kp, &kc);
-        else if (o.precision == DENGO_FP64_FAST) kernels<kFast64>(&kp, &kc);
-        else kernels<kMixed>(&kp, &kc);
+        if (o.precision == DENGO_FP64) kernels<kFaithful>(&kp, &kc, o.kernel_form);
+        else if (o.precision == DENGO_FP64_FAST) kernels<kFast64>(&kp, &kc, o.kernel_form);
+        else kernels<kMixed>(&kp, &kc, o.kernel_form);
         int per_cu = 0;
         DENGO_HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kp, kBlock, 0));
         if (per_cu < 1) per_cu = 1;
@@ -1049,9 +1052,9 @@ struct Host {
         if (dims <= 0) return 0;
         void (*kp)(KParams<Net>) = nullptr;
         void (*kc)(KParams<Net>) = nullptr;
-        if (o.precision == DENGO_FP64) kernels<kFaithful>(&kp, &kc);
-        else if (o.precision == DENGO_FP64_FAST) kernels<kFast64>(&kp, &kc);
-        else kernels<kMixed>(&kp, &kc);
+        if (o.precision == DENGO_FP64) kernels<kFaithful>(&kp, &kc, o.kernel_form);
+        else if (o.precision == DENGO_FP64_FAST) kernels<kFast64>(&kp, &kc, o.kernel_form);
+        else kernels<kMixed>(&kp, &kc, o.kernel_form);
         if (events) DENGO_HIP_OK(hipEventRecord(s->ev0, stream));
         if (o.schedule == DENGO_PERSISTENT) {
             int grid = o.grid;

@@ -28,7 +28,8 @@ class Options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("schedule", ctypes.c_int),
                 ("block", ctypes.c_int), ("grid", ctypes.c_int),
                 ("max_substeps", ctypes.c_int), ("input_is_number", ctypes.c_int),
-                ("reserved", ctypes.c_int * 2), ("order", ctypes.c_void_p),
+                ("kernel_form", ctypes.c_int), ("reserved", ctypes.c_int * 1),
+                ("order", ctypes.c_void_p),
                 ("reserved2", ctypes.c_int * 2)]
 
 
@@ -256,10 +257,12 @@ def _p(a, t=_dp):
 
 
 def options(precision=FP64, schedule=PERSISTENT, grid=0, max_substeps=50000,
-            input_is_number=False, order=None):
+            input_is_number=False, order=None, kernel_form=0):
     """dengo_options.  order: None, or a contiguous int64 CUDA tensor with the
     hand-out order of the cells (see cost_order); the caller keeps it alive
-    until the launch has completed."""
+    until the launch has completed.  kernel_form (FP64-fast only): 0 the
+    register-lean kernel where the network has one that fits, 1 the
+    register-resident kernel (same results bit for bit)."""
     o = Options()
     load().dengo_default_options(ctypes.byref(o))
     o.precision = precision
@@ -267,6 +270,7 @@ def options(precision=FP64, schedule=PERSISTENT, grid=0, max_substeps=50000,
     o.grid = grid
     o.max_substeps = max_substeps
     o.input_is_number = int(bool(input_is_number))
+    o.kernel_form = int(kernel_form)
     if order is not None:
         if not (getattr(order, "is_cuda", False) and order.dtype.itemsize == 8
                 and not order.dtype.is_floating_point and order.is_contiguous()):

@@ -38,7 +38,12 @@ typedef struct dengo_options {
     int grid;             /* blocks for the persistent schedule (0: fill GPU) */
     int max_substeps;     /* accepted-substep cap, reference: 50000 */
     int input_is_number;  /* 1: species given as number densities */
-    int reserved[2];
+    int kernel_form;      /* FP64-fast sweep: 0 the register-lean kernel where
+                             the network has one that fits (two waves per
+                             SIMD), else the register-resident one; 1 always
+                             the register-resident one.  Both give the same
+                             results bit for bit.  Other precisions ignore it. */
+    int reserved[1];
     const long long *order;  /* NULL, or device memory holding the order in
                              which cells are handed out: cell order[0] first,
                              then order[1], ... (a permutation of 0..dims-1).

@@ -58,3 +58,38 @@ def test_no_oracle_in_product():
         if path.endswith((".py", ".hip", ".h", ".cpp")):
             text = open(path).read()
             assert "import oracle" not in text and "liboracle" not in text, path
+
+
+def test_struct_layouts_match_the_headers(tmp_path):
+    """Every struct the bindings mirror has the header's size and field
+    offsets (gcc compiles a probe against include/*.h)."""
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        import pytest
+        pytest.skip("gcc not found")
+    structs = {"dengo_options": runtime.Options, "dengo_cell_outputs": runtime.CellOutputs,
+               "dengo_server_config": runtime.ServerConfig,
+               "dengo_patch_result": runtime.PatchResult,
+               "dengo_server_stats": runtime.ServerStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "dengo_types.h"',
+             '#include "dengo_server.h"', 'int main(void) {']
+    for c, py in structs.items():
+        lines.append('printf("%s sizeof %%zu\\n", sizeof(%s));' % (c, c))
+        for f, _ in py._fields_:
+            lines.append('printf("%s %s %%zu\\n", offsetof(%s, %s));' % (c, f, c, f))
+    lines += ["return 0;", "}"]
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    got = {}
+    for line in out.splitlines():
+        c, f, v = line.split()
+        got[(c, f)] = int(v)
+    for c, py in structs.items():
+        assert got[(c, "sizeof")] == ctypes.sizeof(py), c
+        for f, _ in py._fields_:
+            assert got[(c, f)] == getattr(py, f).offset, (c, f)

@@ -84,15 +84,20 @@ def test_generated_evolve_bitwise(gen, schedule):
     assert got["ttot"] == ref["ttot"] and got["dt"] == ref["dt"]
 
 
-@pytest.mark.parametrize("precision", [runtime.FP64_FAST, runtime.MIXED])
-def test_generated_fast_sweeps_agree_with_faithful(gen, precision):
+@pytest.mark.parametrize("precision,form", [(runtime.FP64_FAST, 0), (runtime.FP64_FAST, 1),
+                                            (runtime.MIXED, 0)])
+def test_generated_fast_sweeps_agree_with_faithful(gen, precision, form):
+    """form 0 runs the FP64-fast sweep as the register-lean kernel where the
+    network's compiled and fits (fused reaction loop), form 1 as the
+    register-resident one."""
     name, _, s = gen
-    print("generated fast sweeps: %s precision %d" % (name, precision), flush=True)
+    print("generated fast sweeps: %s precision %d form %d" % (name, precision, form), flush=True)
     x, _ = _cells(name, 2000, 63)
     rt, at = tolerances(x)
     z = -1.0 if name == "primgen" else 3.0
     ref = s.evolve(x, rt, at, 1e11, z=z, precision=runtime.FP64, max_substeps=3000)
-    got = s.evolve(x, rt, at, 1e11, z=z, precision=precision, max_substeps=3000)
+    got = s.evolve(x, rt, at, 1e11, z=z, precision=precision, max_substeps=3000,
+                   kernel_form=form)
     assert np.mean(got["status"] == ref["status"]) >= 0.97
     done = (got["status"] == 0) & (ref["status"] == 0)
     rel = np.abs(got["state"] - ref["state"]) / np.maximum(np.abs(ref["state"]), 1e-300)

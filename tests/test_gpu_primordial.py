@@ -283,3 +283,36 @@ def test_config1_16cube_bitwise(solver, prim_tables):
     for k in ("state", "atol", "status", "nsub", "nrej", "nsweep"):
         assert np.array_equal(r[k], o[k]), k
     assert r["ttot"] == o["ttot"]
+
+
+@pytest.mark.parametrize("schedule", [runtime.PER_CELL, runtime.PERSISTENT])
+def test_fp64_fast_kernel_forms_bitwise(solver, prim_tables, schedule):
+    """The FP64-fast sweep's two kernels, the register-lean one (the default,
+    two waves per SIMD, cold per-cell data in LDS) and the register-resident
+    one (kernel_form=1), evaluate the same expressions in the same order: on
+    the golden cells (capped cells included) and on 4096 seeded cells of
+    which a third carry their own rtol rows, every output bit for bit."""
+    g = np.load(GOLD)
+    inp, _, _ = primordial_ics(16 ** 3, seed=77)
+    rtol, atol = tolerances(inp)
+    rng = np.random.default_rng(77)
+    own = rng.random(len(inp)) < 1 / 3
+    own[0] = False
+    rtol[own] = 10.0 ** rng.uniform(-11, -5, size=(int(own.sum()), rtol.shape[1]))
+    cases = [("golden cells", g["input"], g["rtol"], g["atol"], float(g["dtf"])),
+             ("4096 cells, own rtol rows", inp, rtol, atol, 1e12)]
+    for name, i, rt, at, dtf in cases:
+        print("fp64fast kernel forms: %s, schedule %d" % (name, schedule), flush=True)
+        lean = solver.evolve(i, rt, at, dtf, precision=runtime.FP64_FAST, schedule=schedule)
+        res = solver.evolve(i, rt, at, dtf, precision=runtime.FP64_FAST, schedule=schedule,
+                            kernel_form=1)
+        _same(lean, res)
+        assert lean["ttot"] == res["ttot"] and lean["dt"] == res["dt"]
+    assert (g["libm_status"] == 2).sum() >= 5
+
+
+def test_kernel_form_rejected(solver):
+    inp, _, _ = primordial_ics(4, seed=1)
+    rtol, atol = tolerances(inp)
+    with pytest.raises(RuntimeError, match="kernel_form"):
+        solver.evolve(inp, rtol, atol, 1e10, precision=runtime.FP64_FAST, kernel_form=2)
