@@ -288,7 +288,8 @@ def main():
                     help="comma-separated extra variants timed once (empty: none)")
     ap.add_argument("--no-percell", action="store_true", help="skip the plain per-cell kernel")
     ap.add_argument("--no-cost-ordered", action="store_true",
-                    help="skip the ordered reruns (cost and density) of the oxygen and dense legs")
+                    help="skip the ordered reruns (cost and density) of the headline, oxygen and "
+                         "dense legs")
     ap.add_argument("--cpu-cells", type=int, default=4096)
     ap.add_argument("--cpu-repeats", type=int, default=5,
                     help="CPU baseline runs (the median is reported)")
@@ -329,6 +330,8 @@ def main():
     from dengo_amd.networks import primordial_network, primordial_tables
 
     dev = torch.device("cuda:0")
+    fp64_measured = runtime.fp64_fma_rate(100000) * 1e12
+    log("fp64 vector FMA rate probe: %.1f TFLOP/s" % (fp64_measured / 1e12))
     names, tab = primordial_tables()
     with np.errstate(all="ignore"):
         sweep_ops = flopcount.sweep_flops(primordial_network())
@@ -343,13 +346,14 @@ def main():
     d_atol = torch.from_numpy(atol).to(dev)
     stream = torch.cuda.current_stream()
 
-    def run(variant, x, at, outputs=None):
+    def run(variant, x, at, outputs=None, order=None):
         prec = (runtime.MIXED if variant.startswith("mixed") else
                 runtime.FP64_FAST if variant.startswith("fp64fast") else runtime.FP64)
         sched = runtime.PER_CELL if variant.endswith("percell") else runtime.PERSISTENT
         form = 1 if "_resident" in variant else 0
         tmin, _ = solver.evolve_device(x, d_rtol, at, 1e12, outputs=outputs, stream=stream,
-                                       precision=prec, schedule=sched, kernel_form=form)
+                                       precision=prec, schedule=sched, kernel_form=form,
+                                       order=order)
         return tmin, runtime.load().dengo_last_kernel_ms()
 
     if args.calib_copy_mib:
@@ -451,6 +455,21 @@ def main():
         extra[v].update(rec)
         extra[v]["bitwise_equal_to_" + args.variant] = bool(torch.equal(xe, xh))
         log("%s: %.1f ms kernel" % (v, ms_e))
+    if not args.no_cost_ordered and not args.variant.endswith("percell"):
+        # the headline sweep with the costly cells handed out first, by the
+        # faithful run's per-cell sweep counts (the order a caller builds
+        # from its previous solve of similar cells); the timed interval
+        # includes the order's gather and scatters
+        order = runtime.cost_order(outs["nsweep"])
+        xe, ae = d_inp.clone(), d_atol.clone()
+        _, ms_e = run(args.variant, xe, ae, order=order)
+        v = args.variant + "_cost_ordered"
+        extra[v] = {"kernel_ms": ms_e, "cell_solves_per_s": n / (ms_e * 1e-3),
+                    "order": "costly first by the per-cell sweep counts of the fp64_persistent "
+                             "run of the same cells",
+                    "bitwise_equal_to_unordered": bool(torch.equal(xe, xh))}
+        log("%s: %.1f ms kernel" % (v, ms_e))
+        del xe, ae, order
     del xs, ats
 
     # roofline: algorithmic FLOPs from the faithful sweep counts
@@ -463,6 +482,9 @@ def main():
             "matrix peaks are both 78.6 TFLOP/s; this kernel issues no MFMA)",
             "achieved": achieved / 1e12,
             "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
+            "peak_measured": fp64_measured / 1e12, "frac_of_measured": achieved / fp64_measured,
+            "peak_measured_source": "dengo_measure_fp64_fma_rate: one launch of independent "
+            "v_fma_f64 chains on this device (16 per lane, 8 x 256 threads per CU)",
             "traffic": (HEADLINE_TRAFFIC_BYTES if args.variant == "fp64fast_persistent"
                         and n == 128 ** 3 else None),
             "traffic_unit": "bytes per launch (calibrated PMC FETCH_SIZE + WRITE_SIZE)",

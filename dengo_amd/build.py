@@ -14,7 +14,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libdengo_hip.so")
 SOURCES = ["dengo_common.hip", "dengo_primordial.hip", "dengo_oxygen.hip", "dengo_synthetic.hip",
-           "dengo_server.hip", "dengo_graph.hip", "dengo_h5.hip", "dengo_noalloc.hip"]
+           "dengo_server.hip", "dengo_graph.hip", "dengo_h5.hip", "dengo_noalloc.hip",
+           "dengo_probe.hip"]
 HEADERS = ["dengo_device.h", "be_driver.h", "be_fast.h", "lean_sweep.h", "gen_net.h", "ma_net.h", "dengo_solver.h", "net_oxygen.h",
            "big_net.h", "big_program.h", "arena.h"]
 

@@ -151,6 +151,7 @@ SIGNATURES = dict(COMMON_SIGNATURES)
 for _n in NETWORKS:
     SIGNATURES.update(_network_signatures(_n))
 SIGNATURES.update({
+    "dengo_measure_fp64_fma_rate": (ctypes.c_int, [ctypes.c_int, _dp]),
     "synthetic_setup_data": (ctypes.c_void_p, [ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, _dp]),
     "synthetic_free_data": (None, [ctypes.c_void_p]),
     "dengo_evolve_synthetic": (ctypes.c_double, [ctypes.c_double, _dp, _dp, _dp, _dp, _dp,
@@ -250,6 +251,14 @@ def last_error(lib=None):
 def _check(rc, what):
     if rc != 0:
         raise RuntimeError("%s failed: %s" % (what, last_error()))
+
+
+def fp64_fma_rate(iters=100000):
+    """Measured FP64 vector FMA rate of the current device (TFLOP/s):
+    dengo_measure_fp64_fma_rate, one timed launch of independent FMA chains."""
+    r = ctypes.c_double(0.0)
+    _check(load().dengo_measure_fp64_fma_rate(int(iters), ctypes.byref(r)), "fp64 rate probe")
+    return r.value
 
 
 def _p(a, t=_dp):

@@ -76,6 +76,13 @@ const char *dengo_last_error(void);
 /* Backend extension: kernel time of the last synchronising evolve call, from
  * HIP events on the launch stream (ms). */
 double dengo_last_kernel_ms(void);
+/* Backend extension (no counterpart in the reference): the FP64 vector FMA
+ * rate of the current device in TFLOP/s, from one timed launch of
+ * independent v_fma_f64 chains (iters FMAs per chain, 16 chains per lane,
+ * eight 256-thread workgroups per CU).  The lane-per-cell sweeps' compute
+ * ceiling, measured beside the published peak.  Allocates and frees its own
+ * buffer; synchronises. */
+int dengo_measure_fp64_fma_rate(int iters, double *tflops);
 
 #ifdef __cplusplus
 }
