@@ -125,8 +125,15 @@ template <class L>
 DENGO_D void begin_be(LeanCell<L> &c)
 {
     using C = LeanCell<L>;
+    // all reads before the writes: one LDS round trip, not one per entry
+    double at[L::N], inv[L::N];
 #pragma unroll
-    for (int i = 0; i < L::N; ++i) c.at(C::L_ATOL + i) *= c.at(C::L_INV + i);
+    for (int i = 0; i < L::N; ++i) {
+        at[i] = c.at(C::L_ATOL + i);
+        inv[i] = c.at(C::L_INV + i);
+    }
+#pragma unroll
+    for (int i = 0; i < L::N; ++i) c.at(C::L_ATOL + i) = at[i] * inv[i];
     c.sweep = 0;
 }
 
@@ -149,6 +156,12 @@ DENGO_D int sweep(LeanCell<L> &c, const KParams<typename L::Net> &p)
     const auto &env = p.env;
     const double(&u)[N] = c.u;
     c.nsweep++;
+    // the column scalings, read while the table rows load
+    double scv[M], invv[M];
+#pragma unroll
+    for (int j = 0; j < M; ++j) scv[j] = c.at(C::L_SC + j);
+#pragma unroll
+    for (int j = 0; j < M; ++j) invv[j] = c.at(C::L_INV + j);
     double T, Tdef, rmd;
     const double2 *row;
     bool ok;
@@ -181,7 +194,7 @@ DENGO_D int sweep(LeanCell<L> &c, const KParams<typename L::Net> &p)
                     part = part * u[L::slot(R, CV(q))];
             });
             constexpr int col = L::slot(R, CV(j));
-            part = part * c.at(C::L_SC + col);
+            part = part * scv[col];
             static_for<0, GE>([&](auto s) {
                 constexpr int v = L::nu(R, CV(s));
                 if constexpr (v != 0) A[CV(s)][col] += (double)v * part;
@@ -196,7 +209,7 @@ DENGO_D int sweep(LeanCell<L> &c, const KParams<typename L::Net> &p)
         const double dt = c.dt;
         static_for<0, M>([&](auto ii) {
             constexpr int i = CV(ii);
-            const double inv = c.at(C::L_INV + i);
+            const double inv = invv[i];
             const double dtinv = dt * inv;
             y[i] = (u[i] - u0(c, i)) * inv - dtinv * f[i];
             static_for<0, M>([&](auto jj) {
@@ -286,24 +299,28 @@ DENGO_D int sweep(LeanCell<L> &c, const KParams<typename L::Net> &p)
         const double w = -dtinv * rmd;
         yge = (u[GE] - u0(c, GE)) * inv - dtinv * fge;
 #pragma unroll
-        for (int j = 0; j < M; ++j) yge = yge - (w * g[j] * c.at(C::L_SC + j)) * y[j];
+        for (int j = 0; j < M; ++j) yge = yge - (w * g[j] * scv[j]) * y[j];
     }
 
     // u -= scale * y; max-norm test in scaled units (first unconverged entry)
-    // the whole rtol row first: loads inside the test's short-circuit would
-    // each wait for the one before
+    // the whole rtol row and the scaled atol first, and the test without
+    // branches: loads inside the test's short-circuit would each wait for
+    // the one before.  The first entry that fails decides, as in the
+    // short-circuit: unsolved if |y| exceeds its bound, else NaN if u is NaN.
     const double *rtrow = p.rtol + ((c.flags & kOwnRtol) ? c.id * N : 0);
-    double rtol[N];
+    double rtol[N], atol[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) rtol[i] = rtrow[L::ext(i)];
+#pragma unroll
+    for (int i = 0; i < N; ++i) atol[i] = c.at(C::L_ATOL + i);
     bool unsolved = false, nan = false;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
         const double yi = i < M ? y[i] : yge;
-        const double inv = c.at(C::L_INV + i);
-        c.u[i] = c.u[i] - c.at(C::L_SC + i) * yi;
+        const double inv = i < M ? invv[i] : c.at(C::L_INV + i);
+        c.u[i] = c.u[i] - (i < M ? scv[i] : c.at(C::L_SC + i)) * yi;
         if (!unsolved && !nan) {
-            if (fabs(yi) > (c.at(C::L_ATOL + i) + rtol[i] * fabs(c.u[i] * inv))) unsolved = true;
+            if (fabs(yi) > (atol[i] + rtol[i] * fabs(c.u[i] * inv))) unsolved = true;
             else if (c.u[i] != c.u[i]) nan = true;
         }
     }
@@ -320,8 +337,17 @@ DENGO_D bool end_be(LeanCell<L> &c, int code, const KParams<typename L::Net> &p)
     constexpr int N = L::N;
     using C = LeanCell<L>;
     int rv = code == SW_CONVERGED ? 0 : 1;
+    {
+        // all reads before the writes: one LDS round trip, not one per entry
+        double at[N], sc[N];
 #pragma unroll
-    for (int i = 0; i < N; ++i) c.at(C::L_ATOL + i) *= c.at(C::L_SC + i);
+        for (int i = 0; i < N; ++i) {
+            at[i] = c.at(C::L_ATOL + i);
+            sc[i] = c.at(C::L_SC + i);
+        }
+#pragma unroll
+        for (int i = 0; i < N; ++i) c.at(C::L_ATOL + i) = at[i] * sc[i];
+    }
 #pragma unroll
     for (int i = 0; i < N; ++i)
         if (c.u[i] < 0 || c.u[i] != c.u[i]) rv = 1;
