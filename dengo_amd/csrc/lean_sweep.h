@@ -146,6 +146,15 @@ DENGO_D double u0(const LeanCell<L> &c, int i)
     return ((c.flags & kFloored) && v < 1e-25) ? 1e-25 : v;
 }
 
+// Whether the network's unknowns are in the caller's column order.
+template <class L>
+constexpr bool ext_is_identity()
+{
+    for (int i = 0; i < L::N; ++i)
+        if (L::ext(i) != i) return false;
+    return true;
+}
+
 // One Newton sweep (be_fast.h sweep_fast<Net, double>, same equations).
 template <class L>
 DENGO_D int sweep(LeanCell<L> &c, const KParams<typename L::Net> &p)
@@ -277,6 +286,29 @@ DENGO_D int sweep(LeanCell<L> &c, const KParams<typename L::Net> &p)
             });
         }
     });
+    // the convergence test's rtol row and scaled atol, loaded here so that
+    // the back substitution and the energy row cover their latency (a load
+    // inside the test's short-circuit would wait there, once per entry)
+    const double *rtrow = p.rtol + ((c.flags & kOwnRtol) ? c.id * N : 0);
+    double rtol[N], atol[N];
+    if constexpr (ext_is_identity<L>() && N % 2 == 0) {
+        // in pairs, one 16-byte load each: the first entry's test always
+        // runs, so a separate load of the second would sink into its branch
+        // and wait there
+        typedef double pair_t __attribute__((ext_vector_type(2), aligned(8)));
+#pragma unroll
+        for (int i = 0; i < N; i += 2) {
+            const pair_t v = *reinterpret_cast<const pair_t *>(rtrow + i);
+            rtol[i] = v.x;
+            rtol[i + 1] = v.y;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) rtol[i] = rtrow[L::ext(i)];
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) atol[i] = c.at(C::L_ATOL + i);
+
     // back substitution (block_lu_apply)
     static_for<0, M>([&](auto rr) {
         constexpr int i = M - 1 - CV(rr);
@@ -302,17 +334,8 @@ DENGO_D int sweep(LeanCell<L> &c, const KParams<typename L::Net> &p)
         for (int j = 0; j < M; ++j) yge = yge - (w * g[j] * scv[j]) * y[j];
     }
 
-    // u -= scale * y; max-norm test in scaled units (first unconverged entry)
-    // the whole rtol row and the scaled atol first, and the test without
-    // branches: loads inside the test's short-circuit would each wait for
-    // the one before.  The first entry that fails decides, as in the
-    // short-circuit: unsolved if |y| exceeds its bound, else NaN if u is NaN.
-    const double *rtrow = p.rtol + ((c.flags & kOwnRtol) ? c.id * N : 0);
-    double rtol[N], atol[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) rtol[i] = rtrow[L::ext(i)];
-#pragma unroll
-    for (int i = 0; i < N; ++i) atol[i] = c.at(C::L_ATOL + i);
+    // u -= scale * y; max-norm test in scaled units (first unconverged entry
+    // decides: unsolved if |y| exceeds its bound, else NaN if u is NaN)
     bool unsolved = false, nan = false;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
