@@ -146,15 +146,6 @@ DENGO_D double u0(const LeanCell<L> &c, int i)
     return ((c.flags & kFloored) && v < 1e-25) ? 1e-25 : v;
 }
 
-// Whether the network's unknowns are in the caller's column order.
-template <class L>
-constexpr bool ext_is_identity()
-{
-    for (int i = 0; i < L::N; ++i)
-        if (L::ext(i) != i) return false;
-    return true;
-}
-
 // One Newton sweep (be_fast.h sweep_fast<Net, double>, same equations).
 template <class L>
 DENGO_D int sweep(LeanCell<L> &c, const KParams<typename L::Net> &p)
@@ -291,20 +282,22 @@ DENGO_D int sweep(LeanCell<L> &c, const KParams<typename L::Net> &p)
     // inside the test's short-circuit would wait there, once per entry)
     const double *rtrow = p.rtol + ((c.flags & kOwnRtol) ? c.id * N : 0);
     double rtol[N], atol[N];
-    if constexpr (ext_is_identity<L>() && N % 2 == 0) {
-        // in pairs, one 16-byte load each: the first entry's test always
-        // runs, so a separate load of the second would sink into its branch
-        // and wait there
+    {
+        // the row in the caller's column order, in 16-byte pairs (for odd N
+        // the last pair overlaps the one before): the first entry's test
+        // always runs, so a separate load of a later entry would sink into
+        // that entry's branch and wait there
         typedef double pair_t __attribute__((ext_vector_type(2), aligned(8)));
+        double rc[N];
 #pragma unroll
-        for (int i = 0; i < N; i += 2) {
-            const pair_t v = *reinterpret_cast<const pair_t *>(rtrow + i);
-            rtol[i] = v.x;
-            rtol[i + 1] = v.y;
+        for (int j = 0; j < N; j += 2) {
+            const int q = j + 1 < N ? j : N - 2;
+            const pair_t v = *reinterpret_cast<const pair_t *>(rtrow + q);
+            rc[q] = v.x;
+            rc[q + 1] = v.y;
         }
-    } else {
 #pragma unroll
-        for (int i = 0; i < N; ++i) rtol[i] = rtrow[L::ext(i)];
+        for (int i = 0; i < N; ++i) rtol[i] = rc[L::ext(i)];
     }
 #pragma unroll
     for (int i = 0; i < N; ++i) atol[i] = c.at(C::L_ATOL + i);
