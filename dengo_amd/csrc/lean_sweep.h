@@ -15,11 +15,16 @@
 // * u0, the state at the start of the BE call, is not stored: it is prev
 //   after a rejected substep or at the cell's start, and prev floored at
 //   1e-25 after an accepted one, so one flag per cell recovers it exactly;
-// * rtol is read before the convergence test in one batch of loads, from the
-//   first cell's row when the cell's own row holds the same bits (the usual
-//   case: one rtol for the whole call, as the emitted main sets it), so that
-//   every such lane of a wave loads the same line, which stays in L1; from
-//   the cell's own row otherwise;
+// * rtol is read in one batch of 16-byte loads before the back substitution,
+//   from the first cell's row when the cell's own row holds the same bits
+//   (the usual case: one rtol for the whole call, as the emitted main sets
+//   it), so that every such lane of a wave loads the same line, which stays
+//   in L1; from the cell's own row otherwise;
+// * loads are issued well before their use and never inside the convergence
+//   test's branches: the LDS values of a phase are read together, and the
+//   rtol loads go out ahead of the energy row's table loads, because a wait
+//   on the load counter covers every older load (waits placed just after
+//   their loads cost 12% of the kernel's time);
 // * the species rows are mass action (dengo/chemical_network.py:176-180,
 //   dengo/reaction_classes.py:148-165): each reaction's rate is interpolated
 //   and used at once for its flux and its Jacobian partials; the energy row
