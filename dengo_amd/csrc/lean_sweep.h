@@ -20,11 +20,13 @@
 //   (the usual case: one rtol for the whole call, as the emitted main sets
 //   it), so that every such lane of a wave loads the same line, which stays
 //   in L1; from the cell's own row otherwise;
-// * loads are issued well before their use and never inside the convergence
-//   test's branches: the LDS values of a phase are read together, and the
-//   rtol loads go out ahead of the energy row's table loads, because a wait
-//   on the load counter covers every older load (waits placed just after
-//   their loads cost 12% of the kernel's time);
+// * loads are issued well before their use: the LDS values of a phase are
+//   read together, and the rtol loads go out ahead of the energy row's table
+//   loads, because a wait on the load counter covers every older load (waits
+//   placed just after their loads cost 12% of the kernel's time);
+// * the convergence test has no branches: one bit per entry, the lowest set
+//   bit decides (a branch per entry cost about twenty scalar exec-mask
+//   instructions each);
 // * the species rows are mass action (dengo/chemical_network.py:176-180,
 //   dengo/reaction_classes.py:148-165): each reaction's rate is interpolated
 //   and used at once for its flux and its Jacobian partials; the energy row
@@ -317,6 +319,20 @@ DENGO_D int sweep(LeanCell<L> &c, const KParams<typename L::Net> &p)
         y[i] = y[i] * A[i][i];
     });
 
+    // the species entries' convergence test (|y| against atol + rtol |u|,
+    // scaled units) as one bit per entry, before the energy row, whose
+    // registers then hold none of the species' tolerances; the updated
+    // values are formed again below by the same operations
+    static_assert(GE == N - 1 && M == N - 1, "the energy is the last unknown");
+    unsigned ov = 0u, nn = 0u;
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        const double un = u[i] - scv[i] * y[i];
+        const bool o = fabs(y[i]) > (atol[i] + rtol[i] * fabs(un * invv[i]));
+        ov |= o ? (1u << i) : 0u;
+        nn |= (!o && un != un) ? (1u << i) : 0u;
+    }
+
     // the energy's Newton step from the energy row: the cooling sum over the
     // mass density and its gradient by species (the BE matrix's energy row)
     double yge;
@@ -332,19 +348,21 @@ DENGO_D int sweep(LeanCell<L> &c, const KParams<typename L::Net> &p)
         for (int j = 0; j < M; ++j) yge = yge - (w * g[j] * scv[j]) * y[j];
     }
 
-    // u -= scale * y; max-norm test in scaled units (first unconverged entry
-    // decides: unsolved if |y| exceeds its bound, else NaN if u is NaN)
-    bool unsolved = false, nan = false;
+    // u -= scale * y, and the energy's test bit; the first entry whose test
+    // fails decides, as in the reference's loop: unsolved if |y| exceeds its
+    // bound, else NaN if u is NaN
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-        const double yi = i < M ? y[i] : yge;
-        const double inv = i < M ? invv[i] : c.at(C::L_INV + i);
-        c.u[i] = c.u[i] - (i < M ? scv[i] : c.at(C::L_SC + i)) * yi;
-        if (!unsolved && !nan) {
-            if (fabs(yi) > (atol[i] + rtol[i] * fabs(c.u[i] * inv))) unsolved = true;
-            else if (c.u[i] != c.u[i]) nan = true;
-        }
+    for (int i = 0; i < M; ++i) c.u[i] = c.u[i] - scv[i] * y[i];
+    {
+        const double inv = c.at(C::L_INV + GE);
+        c.u[GE] = c.u[GE] - c.at(C::L_SC + GE) * yge;
+        const bool o = fabs(yge) > (atol[GE] + rtol[GE] * fabs(c.u[GE] * inv));
+        ov |= o ? (1u << GE) : 0u;
+        nn |= (!o && c.u[GE] != c.u[GE]) ? (1u << GE) : 0u;
     }
+    const unsigned first = (ov | nn) & (0u - (ov | nn));
+    const bool unsolved = (ov & first) != 0u;
+    const bool nan = (nn & first) != 0u;
     if (nan) return SW_FAILED;
     if (!unsolved) return SW_CONVERGED;
     c.sweep++;
