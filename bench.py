@@ -43,9 +43,9 @@ PEAK_FP64 = 78.6e12      # MI355X FP64 (vector and matrix) TFLOP/s
 # HBM bytes per launch of the FP64-fast persistent kernel on the 128^3
 # primordial workload: FETCH_SIZE + WRITE_SIZE from two rocprofv3 --pmc runs
 # of this script (--calib-copy-mib 1024), each scaled by the counter's
-# reading for the 1 GiB device copy: 1,037,079 + 622,250 KiB (means of the
+# reading for the 1 GiB device copy: 1,037,060 + 622,401 KiB (means of the
 # two launches of each run; kernel k_chem_lean<lean::LeanPrim, true>).
-HEADLINE_TRAFFIC_BYTES = 1659329 * 1024
+HEADLINE_TRAFFIC_BYTES = 1659461 * 1024
 PEAK_FP32 = 157.3e12     # MI355X FP32 vector
 
 
