@@ -189,15 +189,19 @@ DENGO_D int sweep_fp64(Cell<Net> &c, const KParams<Net> &p, int &kind)
     gauss_elim<N>(J, f);
 #pragma unroll
     for (int i = 0; i < N; ++i) c.u[i] -= f[i];
-    // max-norm test in scaled units, stopping at the first unconverged entry
-    bool unsolved = false, nan = false;
+    // max-norm test in scaled units: the first entry that fails decides
+    // (unsolved if |s| exceeds its bound, else NaN if u is NaN), as the
+    // reference's loop stops there; one bit per entry, no branch per entry
+    unsigned ov = 0u, nn = 0u;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-        if (!unsolved && !nan) {
-            if (fabs(f[i]) > (c.atol[i] + c.rtol[i] * fabs(c.u[i]))) unsolved = true;
-            else if (c.u[i] != c.u[i]) nan = true;
-        }
+        const bool o = fabs(f[i]) > (c.atol[i] + c.rtol[i] * fabs(c.u[i]));
+        ov |= o ? (1u << i) : 0u;
+        nn |= (!o && c.u[i] != c.u[i]) ? (1u << i) : 0u;
     }
+    const unsigned first = (ov | nn) & (0u - (ov | nn));
+    const bool unsolved = (ov & first) != 0u;
+    const bool nan = (nn & first) != 0u;
     if (nan) {
         kind = FK_NAN;
         return SW_FAILED;

@@ -479,17 +479,20 @@ DENGO_D int sweep_fast(Cell<Net> &c, const KParams<Net> &p)
     double yge = g[GE];
 #pragma unroll
     for (int j = 0; j < M; ++j) yge = yge - age[j] * y[j];
-    // u -= scale * y; max-norm test in scaled units (first unconverged entry)
-    bool unsolved = false, nan = false;
+    // u -= scale * y; max-norm test in scaled units: the first entry that
+    // fails decides (one bit per entry, no branch per entry)
+    unsigned ov = 0u, nn = 0u;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
         const double yi = i < M ? y[i] : yge;
         c.u[i] = c.u[i] - c.sc[i] * yi;
-        if (!unsolved && !nan) {
-            if (fabs(yi) > (c.atol[i] + c.rtol[i] * fabs(c.u[i] * c.inv[i]))) unsolved = true;
-            else if (c.u[i] != c.u[i]) nan = true;
-        }
+        const bool o = fabs(yi) > (c.atol[i] + c.rtol[i] * fabs(c.u[i] * c.inv[i]));
+        ov |= o ? (1u << i) : 0u;
+        nn |= (!o && c.u[i] != c.u[i]) ? (1u << i) : 0u;
     }
+    const unsigned first = (ov | nn) & (0u - (ov | nn));
+    const bool unsolved = (ov & first) != 0u;
+    const bool nan = (nn & first) != 0u;
     if (nan) return SW_FAILED;
     if (!unsolved) return SW_CONVERGED;
     c.sweep++;
