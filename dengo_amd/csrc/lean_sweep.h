@@ -285,15 +285,14 @@ DENGO_D int sweep(LeanCell<L> &c, const KParams<typename L::Net> &p)
         }
     });
     // the convergence test's rtol row and scaled atol, loaded here so that
-    // the back substitution and the energy row cover their latency (a load
-    // inside the test's short-circuit would wait there, once per entry)
+    // the back substitution covers their latency, and ahead of the energy
+    // row's table loads (a wait on the load counter covers every older load)
     const double *rtrow = p.rtol + ((c.flags & kOwnRtol) ? c.id * N : 0);
     double rtol[N], atol[N];
     {
         // the row in the caller's column order, in 16-byte pairs (for odd N
-        // the last pair overlaps the one before): the first entry's test
-        // always runs, so a separate load of a later entry would sink into
-        // that entry's branch and wait there
+        // the last pair overlaps the one before), permuted into the
+        // network's order at compile time
         typedef double pair_t __attribute__((ext_vector_type(2), aligned(8)));
         double rc[N];
 #pragma unroll
